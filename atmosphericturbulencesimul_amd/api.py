@@ -1,0 +1,280 @@
+"""Python front end of libatsim.so (GPU only, no fallback).
+
+The functions mirror the reference's operators for the wavefront path:
+
+    Context.fft2d            do2dfft / do2dffti      (src/fft/fft.c:1298-1315)
+    Context.permut           permut                  (src/fft/fft.c:570-792)
+    Context.fresnel          Fresnel_propagate_wavefront        (WFpropagate.c:68-152)
+    Context.fresnel_tf_init  Init_Fresnel_propagate_wavefront   (WFpropagate.c:166-261)
+    Context.fresnel_tf_apply Fresnel_propagate_wavefront1       (WFpropagate.c:271-330)
+    Context.fresnel_cube     Fresnel_propagate_cube             (WFpropagate.c:337-401)
+    Context.make_master      make_master_turbulence_screen      (AtmosphericTurbulence.c:643-855)
+    Simulation.frame         one frame of make_AtmosphericTurbulence_wavefront_series
+                             (AtmosphericTurbulence.c:2561-4262)
+
+Tensors are torch CUDA tensors (complex64 fields, float32 screens/outputs).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import AtsimError, SimDesc
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    def __init__(self, device=0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise AtsimError("no GPU visible: the wavefront path runs on the GPU only")
+        self.L = _lib.lib()
+        self.device = device
+        h = C.c_void_p()
+        rc = self.L.atsim_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise AtsimError(f"atsim_ctx_create failed ({rc})")
+        self.h = h
+        self._torch_stream = torch.cuda.ExternalStream(self.L.atsim_stream(h), device=f"cuda:{device}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.atsim_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- util
+    def _chk(self, rc):
+        if rc != 0:
+            raise AtsimError(f"atsim error {rc}: {self.L.atsim_error(self.h).decode()}")
+
+    def sync(self):
+        self._chk(self.L.atsim_sync(self.h))
+
+    @property
+    def stream(self):
+        """torch view of the library's HIP stream"""
+        return self._torch_stream
+
+    def _prep(self, t):
+        """make pending torch work visible to the library stream"""
+        torch = _torch()
+        torch.cuda.current_stream(t.device).synchronize()
+        return t
+
+    def device_ok(self):
+        return bool(self.L.atsim_device_ok(self.h))
+
+    # ---------------------------------------------------------------- FFT
+    def fft2d(self, t, sign=-1):
+        """in place, unnormalised; t complex64 (..., n, n)"""
+        assert t.is_contiguous() and t.shape[-1] == t.shape[-2]
+        n = t.shape[-1]
+        nb = t.numel() // (n * n)
+        self._prep(t)
+        self._chk(self.L.atsim_fft2d(self.h, _ptr(t), n, nb, sign))
+        self.sync()
+        return t
+
+    def permut(self, t):
+        assert t.is_contiguous()
+        n1, n0 = t.shape[-2], t.shape[-1]
+        nb = t.numel() // (n0 * n1)
+        self._prep(t)
+        self._chk(self.L.atsim_permut(self.h, _ptr(t), n0, n1, nb))
+        self.sync()
+        return t
+
+    # ---------------------------------------------------------------- Fresnel
+    def fresnel(self, t_in, dx, z, lam, out=None):
+        torch = _torch()
+        assert t_in.is_contiguous() and t_in.dtype == torch.complex64
+        n = t_in.shape[-1]
+        nb = t_in.numel() // (n * n)
+        out = torch.empty_like(t_in) if out is None else out
+        self._prep(t_in)
+        self._chk(self.L.atsim_fresnel(self.h, _ptr(t_in), _ptr(out), n, nb, dx, z, lam))
+        self.sync()
+        return out
+
+    def fresnel_tf_init(self, n, dx, z, lam, fpmaskrad):
+        torch = _torch()
+        Ct = torch.empty((n, n), dtype=torch.complex64, device=f"cuda:{self.device}")
+        self._chk(self.L.atsim_fresnel_tf_init(self.h, _ptr(Ct), n, dx, z, lam, fpmaskrad))
+        self.sync()
+        return Ct
+
+    def fresnel_tf_apply(self, t_in, Ct, out=None):
+        torch = _torch()
+        n = t_in.shape[-1]
+        nb = t_in.numel() // (n * n)
+        out = torch.empty_like(t_in) if out is None else out
+        self._prep(t_in)
+        self._chk(self.L.atsim_fresnel_tf_apply(self.h, _ptr(t_in), _ptr(out), _ptr(Ct), n, nb))
+        self.sync()
+        return out
+
+    def fresnel_cube(self, t_in, dx, zstart, zend, nz, lam):
+        torch = _torch()
+        n = t_in.shape[-1]
+        amp = torch.empty((nz, n, n), dtype=torch.float32, device=t_in.device)
+        pha = torch.empty_like(amp)
+        self._prep(t_in)
+        self._chk(self.L.atsim_fresnel_cube(self.h, _ptr(t_in), _ptr(amp), _ptr(pha), n, dx, zstart, zend, nz, lam))
+        self.sync()
+        return amp, pha
+
+    # ---------------------------------------------------------------- noise / screens
+    def noise(self, n, seed, stream):
+        torch = _torch()
+        t = torch.empty(n, dtype=torch.complex64, device=f"cuda:{self.device}")
+        self._chk(self.L.atsim_noise_c64(self.h, _ptr(t), n, seed, stream))
+        self.sync()
+        return t
+
+    def make_master(self, noise, outerscale_px, innerscale_px, rlim=0.0, want_second=True):
+        torch = _torch()
+        size = noise.shape[-1]
+        o1 = torch.empty((size, size), dtype=torch.float32, device=noise.device)
+        o2 = torch.empty_like(o1) if want_second else None
+        c1 = C.c_double()
+        c2 = C.c_double()
+        self._prep(noise)
+        self._chk(self.L.atsim_make_master(self.h, _ptr(noise), size, outerscale_px, innerscale_px, rlim,
+                                           _ptr(o1), _ptr(o2), C.byref(c1), C.byref(c2)))
+        self.sync()
+        return o1, o2, c1.value, c2.value
+
+    def strfn(self, s, a, b):
+        out = C.c_double()
+        self._prep(s)
+        self._chk(self.L.atsim_strfn(self.h, _ptr(s), s.shape[-1], a, b, C.byref(out)))
+        return out.value
+
+    def r0_measure(self, s, dpix=50):
+        out = C.c_double()
+        self._prep(s)
+        self._chk(self.L.atsim_r0_measure(self.h, _ptr(s), s.shape[-1], dpix, C.byref(out)))
+        return out.value
+
+    def scale(self, s, factor):
+        self._prep(s)
+        self._chk(self.L.atsim_scale(self.h, _ptr(s), s.numel(), factor))
+        self.sync()
+        return s
+
+
+def superlayers(alt, cn2, bin_m):
+    """atsim_superlayers: super-layer altitudes and layer -> super-layer index"""
+    L_ = _lib.lib()
+    alt = np.ascontiguousarray(alt, np.float64)
+    cn2 = np.ascontiguousarray(cn2, np.float64)
+    n = alt.size
+    sa = np.empty(n)
+    sli = np.empty(n, np.int64)
+    S = L_.atsim_superlayers(alt.ctypes.data_as(C.POINTER(C.c_double)), cn2.ctypes.data_as(C.POINTER(C.c_double)),
+                             n, bin_m, sa.ctypes.data_as(C.POINTER(C.c_double)),
+                             sli.ctypes.data_as(C.POINTER(C.c_long)))
+    if S < 0:
+        raise AtsimError("atsim_superlayers: invalid profile (altitudes must be ascending)")
+    return sa[:S].copy(), sli
+
+
+class Simulation:
+    """Multi-wavelength, multi-direction frame loop on one GPU.
+
+    slayer_alt, sli: super-layer binning (see superlayers()); lambdas (W,);
+    scoeff (W, L); unwrap_mode (W,) 0/1; xpos0, ypos0 (D, L) pixels;
+    vx, vy (L,) pixels per second.
+    """
+
+    def __init__(self, ctx, N, Nout, M, slayer_alt, sli, lambdas, scoeff, unwrap_mode, xpos0, ypos0, vx, vy,
+                 dx, dt, zangle=0.0, bicubic=True, fresnel=True, amplitude=True):
+        self.ctx = ctx
+        L = np.asarray(sli).size
+        W = np.asarray(lambdas).size
+        xpos0 = np.ascontiguousarray(np.asarray(xpos0, np.float64).reshape(-1, L))
+        D = xpos0.shape[0]
+        self._keep = dict(
+            sa=np.ascontiguousarray(slayer_alt, np.float64), sli=np.ascontiguousarray(sli, np.int64),
+            lam=np.ascontiguousarray(lambdas, np.float64),
+            sc=np.ascontiguousarray(np.asarray(scoeff, np.float64).reshape(W, L)),
+            uw=np.ascontiguousarray(unwrap_mode, np.int32), x0=xpos0,
+            y0=np.ascontiguousarray(np.asarray(ypos0, np.float64).reshape(D, L)),
+            vx=np.ascontiguousarray(vx, np.float64), vy=np.ascontiguousarray(vy, np.float64))
+        k = self._keep
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        d = SimDesc()
+        d.N, d.Nout, d.M, d.L, d.W, d.D = N, Nout, M, L, W, D
+        d.dx, d.dt, d.zangle = dx, dt, zangle
+        d.bicubic, d.fresnel, d.amplitude = int(bicubic), int(fresnel), int(amplitude)
+        d.slayer_alt = dp(k["sa"])
+        d.super_layer_index = k["sli"].ctypes.data_as(C.POINTER(C.c_long))
+        d.lambda_ = dp(k["lam"])
+        d.scoeff = dp(k["sc"])
+        d.unwrap_mode = k["uw"].ctypes.data_as(C.POINTER(C.c_int))
+        d.xpos0, d.ypos0, d.vx, d.vy = dp(k["x0"]), dp(k["y0"]), dp(k["vx"]), dp(k["vy"])
+        h = C.c_void_p()
+        ctx._chk(ctx.L.atsim_sim_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.N, self.Nout, self.M, self.L, self.W, self.D = N, Nout, M, L, W, D
+        torch = _torch()
+        dev = f"cuda:{ctx.device}"
+        self.pha = torch.empty((D, W, Nout, Nout), dtype=torch.float32, device=dev)
+        self.amp = torch.empty_like(self.pha)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.atsim_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def master_ptr(self, layer):
+        return self.ctx.L.atsim_sim_master(self.h, layer)
+
+    def set_master(self, layer, m):
+        """m: numpy (M, M) array or torch float32 CUDA tensor"""
+        torch = _torch()
+        if isinstance(m, np.ndarray):
+            a = np.ascontiguousarray(m, np.float32)
+            assert a.size == self.M * self.M
+            self.ctx._chk(self.ctx.L.atsim_sim_set_master(self.h, layer, a.ctypes.data_as(C.c_void_p)))
+        else:
+            assert m.dtype == torch.float32 and m.is_contiguous() and m.numel() == self.M * self.M
+            torch.cuda.current_stream(m.device).synchronize()
+            self.ctx._chk(self.ctx.L.atsim_sim_set_master(self.h, layer, _ptr(m)))
+
+    def frame(self, field=None, sync=True):
+        """advance one time step; returns (pha, amp) views [D, W, Nout, Nout]"""
+        self.ctx._chk(self.ctx.L.atsim_sim_frame(self.h, _ptr(self.pha), _ptr(self.amp), _ptr(field)))
+        if sync:
+            self.ctx.sync()
+        return self.pha, self.amp
+
+    def positions(self):
+        x = np.empty((self.D, self.L))
+        y = np.empty((self.D, self.L))
+        self.ctx._chk(self.ctx.L.atsim_sim_positions(self.h, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     y.ctypes.data_as(C.POINTER(C.c_double))))
+        return x, y
+
+    def gaps(self):
+        return self.ctx.L.atsim_sim_gaps(self.h)

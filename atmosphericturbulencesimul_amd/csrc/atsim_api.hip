@@ -1,0 +1,642 @@
+// atsim_api.hip -- host runtime and C ABI (include/atsim.h) of the wavefront
+// library: contexts, device buffers, transfer-function tables, the per-frame
+// launch sequence of the layer loop.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "atsim.h"
+#include "atsim_kernels.hpp"
+#include "atsim_launch.hpp"
+
+using namespace atsim;
+
+struct atsim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<long, float2*> twiddles;
+};
+
+static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail((ctx), ATSIM_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+static bool pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// e^{-2 pi i m / n}, m < n, built in FP64
+static int get_twiddles(atsim_ctx* c, long n, float2** out) {
+    auto it = c->twiddles.find(n);
+    if (it != c->twiddles.end()) { *out = it->second; return 0; }
+    std::vector<float2> h(n);
+    for (long m = 0; m < n; m++) {
+        double a = -2.0 * M_PI * (double)m / (double)n;
+        h[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    float2* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, sizeof(float2) * n));
+    HIPCHK(c, hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    c->twiddles[n] = d;
+    *out = d;
+    return 0;
+}
+
+// 1D Fresnel transfer factor h(k) = e^{-i coeff kt^2} / n, kt = signed
+// frequency index, coeff = pi z lambda / (dx n)^2 (WFpropagate.c:97-121):
+// h(kx) h(ky) is the reference's quadrant-swapped 2D factor / n^2.
+static void fresnel_table(float2* h, long n, double dx, double z, double lambda) {
+    const double coeff = M_PI * z * lambda / (dx * n) / (dx * n);
+    for (long k = 0; k < n; k++) {
+        const double kt = (double)(k < n / 2 ? k : k - n);
+        const double ang = -coeff * kt * kt;
+        h[k] = make_float2((float)(cos(ang) / n), (float)(sin(ang) / n));
+    }
+}
+
+extern "C" {
+
+int atsim_ctx_create(int device, atsim_ctx** out) {
+    if (!out) return ATSIM_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ATSIM_EHIP;
+    if (device < 0 || device >= ndev) return ATSIM_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return ATSIM_EHIP;
+    atsim_ctx* c = new atsim_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ATSIM_EHIP;
+    }
+    *out = c;
+    return ATSIM_OK;
+}
+
+void atsim_ctx_destroy(atsim_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->twiddles) hipFree(kv.second);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* atsim_error(const atsim_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int atsim_sync(atsim_ctx* c) {
+    if (!c) return ATSIM_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATSIM_OK;
+}
+
+void* atsim_stream(atsim_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int atsim_device_ok(atsim_ctx* c) {
+    if (!c) return 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, c->device) != hipSuccess) return 0;
+    return strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+// ------------------------------------------------------------------ FFT
+int atsim_fft2d(atsim_ctx* c, void* d_data, long n, long nbatch, int sign) {
+    if (!c || !d_data) return ATSIM_EINVAL;
+    if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fft2d: size %ld is not a power of two in [16, 8192]", n);
+    if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fft2d: nbatch %ld out of range", nbatch);
+    if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "fft2d: sign must be -1 or +1");
+    float2* tw;
+    if (int r = get_twiddles(c, n, &tw)) return r;
+    ScreenArgs none{};
+    HIPCHK(c, launch_row(n, OP_FFT, sign, false, (float2*)d_data, nbatch, nullptr, tw, none, c->stream));
+    HIPCHK(c, launch_col(n, OP_FFT, sign, (float2*)d_data, nbatch, nullptr, 1, tw, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_permut(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch) {
+    if (!c || !d_data) return ATSIM_EINVAL;
+    if (n0 <= 0 || n1 <= 0 || (n0 & 1) || (n1 & 1) || nbatch <= 0)
+        return fail(c, ATSIM_EUNSUPPORTED, "permut: device path needs even sizes (got %ld x %ld)", n0, n1);
+    HIPCHK(c, launch_permut((float2*)d_data, n0, n1, nbatch, c->stream));
+    return ATSIM_OK;
+}
+
+// ------------------------------------------------------------------ Fresnel
+static int fresnel_run(atsim_ctx* c, float2* d, long n, long nbatch, const float2* d_tab) {
+    float2* tw;
+    if (int r = get_twiddles(c, n, &tw)) return r;
+    ScreenArgs none{};
+    none.W = 1;
+    HIPCHK(c, launch_row(n, OP_FRESNEL, 0, false, d, nbatch, d_tab, tw, none, c->stream));
+    HIPCHK(c, launch_col(n, OP_FRESNEL, 0, d, nbatch, d_tab, 1, tw, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_fresnel(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbatch, double dx, double z,
+                  double lambda) {
+    if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel: size %ld is not a power of two in [16, 8192]", n);
+    if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fresnel: nbatch %ld out of range", nbatch);
+    if (!(dx > 0.0)) return fail(c, ATSIM_EINVAL, "fresnel: pupil scale must be > 0");
+    std::vector<float2> h(n);
+    fresnel_table(h.data(), n, dx, z, lambda);
+    float2* d_tab = nullptr;
+    HIPCHK(c, hipMallocAsync((void**)&d_tab, sizeof(float2) * n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_tab, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+    if (d_in != d_out)
+        HIPCHK(c, hipMemcpyAsync(d_out, d_in, sizeof(float2) * n * n * nbatch, hipMemcpyDeviceToDevice, c->stream));
+    int r = fresnel_run(c, (float2*)d_out, n, nbatch, d_tab);
+    hipFreeAsync(d_tab, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host table must outlive the copy
+    return r;
+}
+
+int atsim_fresnel_tf_init(atsim_ctx* c, void* d_C, long n, double dx, double z, double lambda, double fpmaskrad) {
+    if (!c || !d_C) return ATSIM_EINVAL;
+    if (n <= 0 || (n & 1)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_tf_init: even size required (got %ld)", n);
+    const double coeff = M_PI * z * lambda / (dx * n) / (dx * n);
+    HIPCHK(c, launch_tf_init((float2*)d_C, n, coeff, fpmaskrad, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_fresnel_tf_apply(atsim_ctx* c, const void* d_in, void* d_out, const void* d_C, long n, long nbatch) {
+    if (!c || !d_in || !d_out || !d_C) return ATSIM_EINVAL;
+    if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_tf_apply: size %ld unsupported", n);
+    if (d_in != d_out)
+        HIPCHK(c, hipMemcpyAsync(d_out, d_in, sizeof(float2) * n * n * nbatch, hipMemcpyDeviceToDevice, c->stream));
+    if (int r = atsim_fft2d(c, d_out, n, nbatch, -1)) return r;
+    HIPCHK(c, launch_cmul((float2*)d_out, (const float2*)d_C, n * n, nbatch, c->stream));
+    return atsim_fft2d(c, d_out, n, nbatch, +1);
+}
+
+int atsim_fresnel_cube(atsim_ctx* c, const void* d_in, float* d_amp, float* d_pha, long n, double dx,
+                       double zstart, double zend, long nz, double lambda) {
+    if (!c || !d_in || !d_amp || !d_pha) return ATSIM_EINVAL;
+    if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_cube: size %ld unsupported", n);
+    if (nz <= 0) return fail(c, ATSIM_EINVAL, "fresnel_cube: nz must be > 0");
+    float2* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, sizeof(float2) * n * n));
+    int r = ATSIM_OK;
+    for (long k = 0; k < nz && r == ATSIM_OK; k++) {
+        const double zp = zstart + (zend - zstart) * k / nz;
+        r = atsim_fresnel(c, d_in, tmp, n, 1, dx, zp, lambda);
+        if (r == ATSIM_OK) {
+            hipError_t e = launch_amp_pha(tmp, d_amp + k * n * n, d_pha + k * n * n, n * n, c->stream);
+            if (e != hipSuccess) r = fail(c, ATSIM_EHIP, "amp_pha: %s", hipGetErrorString(e));
+        }
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    return r;
+}
+
+// ------------------------------------------------------------------ noise
+int atsim_noise_c64(atsim_ctx* c, void* d_out, long n, uint64_t seed, uint32_t stream) {
+    if (!c || !d_out || n <= 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_noise((float2*)d_out, n, seed, stream, c->stream));
+    return ATSIM_OK;
+}
+
+// ------------------------------------------------------------------ screens
+static int reduce_partials(atsim_ctx* c, double* d_part, int nblocks, int nv, std::vector<double>& out) {
+    std::vector<double> h((size_t)nblocks * nv);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d_part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out.assign(nv, 0.0);
+    for (int b = 0; b < nblocks; b++)
+        for (int k = 0; k < nv; k++) out[k] += h[(size_t)b * nv + k];
+    return 0;
+}
+
+int atsim_make_master(atsim_ctx* c, const void* d_noise, long size, double outerscale_px, double innerscale_px,
+                      double rlim, float* d_out1, float* d_out2, double* C1, double* C2) {
+    if (!c || !d_noise || !d_out1) return ATSIM_EINVAL;
+    if (!supported_n(size)) return fail(c, ATSIM_EUNSUPPORTED, "make_master: size %ld unsupported", size);
+    if (!(outerscale_px > 0) || !(innerscale_px > 0)) return fail(c, ATSIM_EINVAL, "make_master: scales must be > 0");
+    const double f0 = 1.0 * size / outerscale_px;                        // AT.c:679
+    const double fi = (5.92 / (2.0 * M_PI)) * size / innerscale_px;       // AT.c:680
+    float2* buf = nullptr;
+    const int nblocks = 1024;
+    double* part = nullptr;
+    HIPCHK(c, hipMalloc(&buf, sizeof(float2) * size * size));
+    if (hipMalloc(&part, sizeof(double) * nblocks * 8) != hipSuccess) {
+        hipFree(buf);
+        return fail(c, ATSIM_ENOMEM, "make_master: allocation failed");
+    }
+    int r = ATSIM_OK;
+    std::vector<double> s;
+    hipError_t e = launch_synth_filter((const float2*)d_noise, buf, size, f0, fi, rlim, c->stream);
+    if (e != hipSuccess) r = fail(c, ATSIM_EHIP, "synth_filter: %s", hipGetErrorString(e));
+    if (r == ATSIM_OK) r = atsim_fft2d(c, buf, size, 1, -1);
+    if (r == ATSIM_OK) {
+        e = launch_lag_sums(buf, size, part, nblocks, c->stream);
+        if (e != hipSuccess) r = fail(c, ATSIM_EHIP, "lag_sums: %s", hipGetErrorString(e));
+    }
+    if (r == ATSIM_OK) r = reduce_partials(c, part, nblocks, 8, s);
+    if (r == ATSIM_OK) {
+        // AT.c:794-847: lags (ii, jj) in {1,2}^2, ii outer; sums ordered (1,1) (1,2) (2,1) (2,2)
+        const double n2 = (double)size * (double)size;
+        const int la[4] = {1, 1, 2, 2}, lb[4] = {1, 2, 1, 2};
+        double v1 = 0.0, v2 = 0.0;
+        for (int q = 0; q < 4; q++) {
+            const double lg = 5.0 / 3.0 * log10(sqrt((double)(la[q] * la[q] + lb[q] * lb[q])));
+            v1 += log10(s[q] / n2) - lg;
+            v2 += log10(s[4 + q] / n2) - lg;
+        }
+        const double c1 = pow(10.0, v1 / 4.0), c2 = pow(10.0, v2 / 4.0);
+        if (C1) *C1 = c1;
+        if (C2) *C2 = c2;
+        e = launch_split_scale(buf, d_out1, d_out2, size * size, 1.0 / sqrt(c1), 1.0 / sqrt(c2), c->stream);
+        if (e != hipSuccess) r = fail(c, ATSIM_EHIP, "split_scale: %s", hipGetErrorString(e));
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(part);
+    hipFree(buf);
+    return r;
+}
+
+int atsim_strfn(atsim_ctx* c, const float* d_screen, long size, long a, long b, double* out) {
+    if (!c || !d_screen || !out || size <= 0) return ATSIM_EINVAL;
+    const int nblocks = 512;
+    double* part = nullptr;
+    HIPCHK(c, hipMalloc(&part, sizeof(double) * nblocks));
+    long aa = ((a % size) + size) % size;
+    hipError_t e = launch_strfn(d_screen, size, aa, b, 1, part, nblocks, c->stream);
+    std::vector<double> s;
+    int r = e == hipSuccess ? reduce_partials(c, part, nblocks, 1, s) : fail(c, ATSIM_EHIP, "strfn launch");
+    hipFree(part);
+    if (r) return r;
+    *out = s[0] / ((double)size * (double)size);
+    return ATSIM_OK;
+}
+
+int atsim_r0_measure(atsim_ctx* c, const float* d_screen, long size, long dpix, double* r0) {
+    if (!c || !d_screen || !r0 || dpix <= 0 || dpix >= size) return ATSIM_EINVAL;
+    const int nblocks = 512;
+    double* part = nullptr;
+    HIPCHK(c, hipMalloc(&part, sizeof(double) * nblocks));
+    hipError_t e = launch_strfn(d_screen, size, dpix, 0, 0, part, nblocks, c->stream);
+    std::vector<double> s;
+    int r = e == hipSuccess ? reduce_partials(c, part, nblocks, 1, s) : fail(c, ATSIM_EHIP, "r0 launch");
+    hipFree(part);
+    if (r) return r;
+    const double cnt = (double)(size - dpix) * (double)size;
+    *r0 = 1.0 * dpix * pow((s[0] / cnt) / 6.88, -3.0 / 5.0);  // AT.c:2269
+    return ATSIM_OK;
+}
+
+int atsim_scale(atsim_ctx* c, float* d_screen, long n, double scale) {
+    if (!c || !d_screen || n <= 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_scale(d_screen, n, scale, c->stream));
+    return ATSIM_OK;
+}
+
+// ------------------------------------------------------------------ layers
+long atsim_superlayers(const double* alt, const double* cn2, long L, double bin, double* sa, long* sli) {
+    if (!alt || !cn2 || !sa || !sli || L <= 0) return ATSIM_EINVAL;
+    for (long i = 1; i < L; i++)
+        if (alt[i] < alt[i - 1]) return ATSIM_EINVAL;
+    std::vector<double> sc(cn2, cn2 + L);
+    long S = L;
+    for (long i = 0; i < L; i++) sa[i] = alt[i];
+    // AT.c:2084-2120: merge the closest pair until every gap exceeds bin
+    for (;;) {
+        double minaltd = alt[L - 1];
+        long index = 0;
+        for (long i = 0; i < S - 1; i++) {
+            const double v = sa[i + 1] - sa[i];
+            if (v < minaltd) { minaltd = v; index = i; }
+        }
+        if (minaltd > bin || S == 1) break;
+        sa[index] = (sc[index] * sa[index] + sc[index + 1] * sa[index + 1]) / (sc[index] + sc[index + 1]);
+        sc[index] = sc[index] + sc[index + 1];
+        for (long i = index + 1; i < S - 1; i++) { sa[i] = sa[i + 1]; sc[i] = sc[i + 1]; }
+        S -= 1;
+    }
+    // AT.c:2126-2150: separators at mid-points, layer -> super-layer
+    const long nsep = S - 1;
+    std::vector<double> sep(nsep > 0 ? nsep : 1);
+    for (long i = 0; i < nsep; i++) sep[i] = 0.5 * (sa[i] + sa[i + 1]);
+    for (long l = 0; l < L; l++) {
+        long index = 0;
+        for (long i = 0; i + 1 < nsep; i++)
+            if (sep[i] < alt[l] && sep[i + 1] > alt[l]) index = i + 1;
+        if (nsep > 0 && alt[l] > sep[nsep - 1]) index = nsep;
+        sli[l] = index;
+    }
+    return S;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ simulation
+struct Group {
+    int lfirst, lcount;  // layers [lfirst, lfirst + lcount)
+    double dz;           // propagation distance before this group (0 for the top group)
+};
+
+struct atsim_sim {
+    atsim_ctx* ctx = nullptr;
+    long N = 0, Nout = 0, M = 0, L = 0, W = 0, D = 0;
+    double dx = 0, dt = 0, zangle = 0;
+    int bicubic = 1, fresnel = 1, amplitude = 1;
+    std::vector<double> lambda, scoeff, xpos, ypos, vx, vy;
+    std::vector<int> unwrap;
+    std::vector<Group> groups;  // top -> bottom
+    bool propagate = false;     // Fresnel passes run this frame sequence
+    float* masters = nullptr;
+    float2* fields = nullptr;
+    float2* tf = nullptr;       // [gap][W][N]
+    float2* tw = nullptr;
+    LayerSample* ls_dev = nullptr;
+    LayerSample* ls_host = nullptr;  // pinned ring
+    hipEvent_t* ls_ev = nullptr;
+    int ls_slot = 0;
+    static constexpr int RING = 8;
+    double* s_dev = nullptr;
+    int* unwrap_dev = nullptr;
+    double* dsave = nullptr;
+    double* bsum = nullptr;
+    double* bin = nullptr;
+    double* ch = nullptr;
+    int* flist_dev = nullptr;
+    int nblockfields = 0;
+    int xs = 0;
+};
+
+static void sim_free(atsim_sim* s) {
+    if (!s) return;
+    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    hipFree(s->masters);
+    hipFree(s->fields);
+    hipFree(s->tf);
+    hipFree(s->ls_dev);
+    if (s->ls_host) hipHostFree(s->ls_host);
+    if (s->ls_ev) {
+        for (int i = 0; i < atsim_sim::RING; i++) hipEventDestroy(s->ls_ev[i]);
+        delete[] s->ls_ev;
+    }
+    hipFree(s->s_dev);
+    hipFree(s->unwrap_dev);
+    hipFree(s->dsave);
+    hipFree(s->bsum);
+    hipFree(s->bin);
+    hipFree(s->ch);
+    hipFree(s->flist_dev);
+    delete s;
+}
+
+extern "C" {
+
+int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
+    if (!c || !d || !out) return ATSIM_EINVAL;
+    *out = nullptr;
+    if (!supported_n(d->N)) return fail(c, ATSIM_EUNSUPPORTED, "sim: N=%ld must be a power of two in [16, 8192]", d->N);
+    if (d->Nout <= 0 || d->Nout > d->N) return fail(c, ATSIM_EINVAL, "sim: need 0 < Nout <= N");
+    if (d->M < d->N) return fail(c, ATSIM_EINVAL, "sim: need N <= M (master %ld, grid %ld)", d->M, d->N);
+    if (d->M > (1L << 20)) return fail(c, ATSIM_EINVAL, "sim: master too large");
+    if (d->L <= 0 || d->W <= 0 || d->D <= 0) return fail(c, ATSIM_EINVAL, "sim: L, W, D must be > 0");
+    if (d->D * d->W > 65535) return fail(c, ATSIM_EINVAL, "sim: too many fields");
+    if (!(d->dx > 0)) return fail(c, ATSIM_EINVAL, "sim: dx must be > 0");
+    if (!d->slayer_alt || !d->super_layer_index || !d->lambda || !d->scoeff || !d->unwrap_mode || !d->xpos0 ||
+        !d->ypos0 || !d->vx || !d->vy)
+        return fail(c, ATSIM_EINVAL, "sim: null descriptor array");
+    for (long l = 1; l < d->L; l++)
+        if (d->super_layer_index[l] < d->super_layer_index[l - 1])
+            return fail(c, ATSIM_EINVAL, "sim: layers must be in ascending altitude order");
+    HIPCHK(c, hipSetDevice(c->device));
+    atsim_sim* s = new atsim_sim();
+    s->ctx = c;
+    s->N = d->N; s->Nout = d->Nout; s->M = d->M; s->L = d->L; s->W = d->W; s->D = d->D;
+    s->dx = d->dx; s->dt = d->dt; s->zangle = d->zangle;
+    s->bicubic = d->bicubic; s->fresnel = d->fresnel; s->amplitude = d->amplitude;
+    s->lambda.assign(d->lambda, d->lambda + d->W);
+    s->scoeff.assign(d->scoeff, d->scoeff + d->W * d->L);
+    s->unwrap.assign(d->unwrap_mode, d->unwrap_mode + d->W);
+    s->xpos.assign(d->xpos0, d->xpos0 + d->D * d->L);
+    s->ypos.assign(d->ypos0, d->ypos0 + d->D * d->L);
+    s->vx.assign(d->vx, d->vx + d->L);
+    s->vy.assign(d->vy, d->vy + d->L);
+    // groups in the order of the layer loop, top -> bottom (AT.c:2649-2692)
+    for (long l = d->L - 1; l >= 0; l--) {
+        if (l == d->L - 1 || d->super_layer_index[l + 1] != d->super_layer_index[l]) {
+            Group g;
+            g.lfirst = (int)l;
+            g.lcount = 1;
+            g.dz = (l == d->L - 1) ? 0.0
+                                   : (d->slayer_alt[d->super_layer_index[l + 1]] - d->slayer_alt[d->super_layer_index[l]]) /
+                                         cos(d->zangle);
+            s->groups.push_back(g);
+        } else {
+            s->groups.back().lfirst = (int)l;
+            s->groups.back().lcount++;
+        }
+    }
+    const long G = (long)s->groups.size() - 1;
+    s->propagate = s->amplitude && s->fresnel && G > 0;
+    const long F = s->D * s->W;
+    const size_t nn = (size_t)s->N * s->N, no = (size_t)s->Nout * s->Nout;
+    int r = ATSIM_OK;
+#define SIMALLOC(ptr, bytes)                                                                       \
+    do {                                                                                           \
+        if (r == ATSIM_OK && hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)                     \
+            r = fail(c, ATSIM_ENOMEM, "sim: cannot allocate %zu bytes for " #ptr, (size_t)(bytes)); \
+    } while (0)
+    SIMALLOC(s->masters, sizeof(float) * s->L * s->M * s->M);
+    if (s->propagate) {
+        SIMALLOC(s->fields, sizeof(float2) * F * nn);
+        SIMALLOC(s->tf, sizeof(float2) * G * s->W * s->N);
+    }
+    SIMALLOC(s->ls_dev, sizeof(LayerSample) * s->D * s->L);
+    SIMALLOC(s->s_dev, sizeof(double) * s->W * s->L);
+    SIMALLOC(s->unwrap_dev, sizeof(int) * s->W);
+    if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
+        r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
+    if (r == ATSIM_OK) {
+        s->ls_ev = new hipEvent_t[atsim_sim::RING];
+        for (int i = 0; i < atsim_sim::RING; i++) hipEventCreateWithFlags(&s->ls_ev[i], hipEventDisableTiming);
+    }
+    std::vector<int> flist;
+    for (long dd = 0; dd < s->D; dd++)
+        for (long w = 0; w < s->W; w++)
+            if (s->unwrap[w] != 0 && s->amplitude) flist.push_back((int)(dd * s->W + w));
+    s->nblockfields = (int)flist.size();
+    s->xs = (int)(s->Nout / 20);
+    if (s->nblockfields > 0) {
+        const size_t nb = (size_t)(s->xs > 0 ? s->xs * s->xs : 1);
+        SIMALLOC(s->dsave, sizeof(double) * F * no);
+        SIMALLOC(s->bsum, sizeof(double) * F * nb * 2);
+        SIMALLOC(s->bin, sizeof(double) * F * nb);
+        SIMALLOC(s->ch, sizeof(double) * F * nb);
+        SIMALLOC(s->flist_dev, sizeof(int) * flist.size());
+    }
+#undef SIMALLOC
+    if (r == ATSIM_OK && hipMemsetAsync(s->masters, 0, sizeof(float) * s->L * s->M * s->M, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "sim: memset failed");
+    if (r == ATSIM_OK) r = get_twiddles(c, s->N, &s->tw);
+    if (r == ATSIM_OK && s->propagate) {
+        std::vector<float2> h((size_t)G * s->W * s->N);
+        for (long g = 1; g <= G; g++)
+            for (long w = 0; w < s->W; w++)
+                fresnel_table(h.data() + ((g - 1) * s->W + w) * s->N, s->N, s->dx, s->groups[g].dz, s->lambda[w]);
+        if (hipMemcpy(s->tf, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            r = fail(c, ATSIM_EHIP, "sim: table upload failed");
+    }
+    if (r == ATSIM_OK) {
+        if (hipMemcpy(s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W, hipMemcpyHostToDevice) != hipSuccess ||
+            (s->nblockfields > 0 &&
+             hipMemcpy(s->flist_dev, flist.data(), sizeof(int) * flist.size(), hipMemcpyHostToDevice) != hipSuccess))
+            r = fail(c, ATSIM_EHIP, "sim: parameter upload failed");
+    }
+    if (r == ATSIM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, ATSIM_EHIP, "sim: init sync failed");
+    if (r != ATSIM_OK) {
+        sim_free(s);
+        return r;
+    }
+    *out = s;
+    return ATSIM_OK;
+}
+
+void atsim_sim_destroy(atsim_sim* s) { sim_free(s); }
+
+float* atsim_sim_master(atsim_sim* s, long layer) {
+    if (!s || layer < 0 || layer >= s->L) return nullptr;
+    return s->masters + (size_t)layer * s->M * s->M;
+}
+
+int atsim_sim_set_master(atsim_sim* s, long layer, const float* src) {
+    if (!s || !src || layer < 0 || layer >= s->L) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(s->masters + (size_t)layer * s->M * s->M, src, sizeof(float) * s->M * s->M, hipMemcpyDefault));
+    return ATSIM_OK;
+}
+
+int atsim_sim_positions(atsim_sim* s, double* xp, double* yp) {
+    if (!s) return ATSIM_EINVAL;
+    if (xp) memcpy(xp, s->xpos.data(), sizeof(double) * s->xpos.size());
+    if (yp) memcpy(yp, s->ypos.data(), sizeof(double) * s->ypos.size());
+    return ATSIM_OK;
+}
+
+long atsim_sim_gaps(atsim_sim* s) { return s ? (long)s->groups.size() - 1 : -1; }
+
+// Catmull-Rom weights for taps -1, 0, 1, 2 (the tensor product is the a_ij
+// polynomial of AT.c:2884-2910); bilinear puts 1-f, f on taps 0, 1.
+static void tap_weights(double f, int bicubic, double w[4]) {
+    if (bicubic) {
+        const double f2 = f * f, f3 = f2 * f;
+        w[0] = 0.5 * (-f3 + 2.0 * f2 - f);
+        w[1] = 0.5 * (3.0 * f3 - 5.0 * f2 + 2.0);
+        w[2] = 0.5 * (-3.0 * f3 + 4.0 * f2 + f);
+        w[3] = 0.5 * (f3 - f2);
+    } else {
+        w[0] = 0.0;
+        w[1] = 1.0 - f;
+        w[2] = f;
+        w[3] = 0.0;
+    }
+}
+
+int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
+    if (!s || !d_pha || !d_amp) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long L = s->L, D = s->D, W = s->W, M = s->M;
+    // positions: AT.c:2706-2755 (cumulative FP64, wrapped into [0, M])
+    LayerSample* slot = s->ls_host + (size_t)s->ls_slot * D * L;
+    HIPCHK(c, hipEventSynchronize(s->ls_ev[s->ls_slot]));
+    for (long d = 0; d < D; d++)
+        for (long l = 0; l < L; l++) {
+            double& x = s->xpos[d * L + l];
+            double& y = s->ypos[d * L + l];
+            x += s->vx[l] * s->dt;
+            y += s->vy[l] * s->dt;
+            while (x < 0) x += 1.0 * M;
+            while (x > 1.0 * M) x -= 1.0 * M;
+            while (y < 0) y += 1.0 * M;
+            while (y > 1.0 * M) y -= 1.0 * M;
+            const double xi = floor(x), yi = floor(y);
+            LayerSample& p = slot[d * L + l];
+            p.x0 = (int)((long)xi % M);
+            p.y0 = (int)((long)yi % M);
+            p.pad0 = p.pad1 = 0;
+            tap_weights(x - xi, s->bicubic, p.wx);
+            tap_weights(y - yi, s->bicubic, p.wy);
+        }
+    HIPCHK(c, hipMemcpyAsync(s->ls_dev, slot, sizeof(LayerSample) * D * L, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(s->ls_ev[s->ls_slot], c->stream));
+    s->ls_slot = (s->ls_slot + 1) % atsim_sim::RING;
+
+    ScreenArgs sa{};
+    sa.masters = s->masters;
+    sa.ls = s->ls_dev;
+    sa.s = s->s_dev;
+    sa.M = M;
+    sa.L = (int)L;
+    sa.W = (int)W;
+    const long F = D * W;
+    const long G = (long)s->groups.size() - 1;
+    if (s->propagate) {
+        for (long g = 0; g < G; g++) {
+            sa.lfirst = s->groups[g].lfirst;
+            sa.lcount = s->groups[g].lcount;
+            const float2* tab = s->tf + (size_t)g * W * s->N;
+            HIPCHK(c, launch_row(s->N, OP_FRESNEL, 0, g == 0, s->fields, F, tab, s->tw, sa, c->stream));
+            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream));
+        }
+    }
+    ExtractArgs a{};
+    a.fields = s->propagate ? s->fields : nullptr;
+    a.field_out = (float2*)d_field;
+    a.pha = d_pha;
+    a.amp = d_amp;
+    a.dsave = s->dsave;
+    a.bsum = s->bsum;
+    a.unwrap_mode = s->unwrap_dev;
+    a.all = sa;
+    a.all.lfirst = 0;
+    a.all.lcount = (int)L;
+    if (s->propagate) {
+        a.last_first = s->groups[G].lfirst;
+        a.last_count = s->groups[G].lcount;
+    } else {
+        a.last_first = 0;
+        a.last_count = (int)L;
+    }
+    a.N = (int)s->N;
+    a.Nout = (int)s->Nout;
+    a.W = (int)W;
+    a.D = (int)D;
+    a.amplitude = s->amplitude;
+    a.xs = s->xs;
+    if (s->nblockfields > 0 && s->xs > 0)
+        HIPCHK(c, hipMemsetAsync(s->bsum, 0, sizeof(double) * F * s->xs * s->xs * 2, c->stream));
+    HIPCHK(c, launch_extract(a, d_field != nullptr, c->stream));
+    if (s->nblockfields > 0) {
+        HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->flist_dev, s->nblockfields, s->xs, c->stream));
+        HIPCHK(c, launch_block_apply(d_pha, s->dsave, s->bin, s->flist_dev, s->nblockfields, (int)s->Nout, s->xs,
+                                     c->stream));
+    }
+    return ATSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+// atsim_fft.hpp -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A length-N transform (N = 2^n, 16 <= N <= 8192) is split into S stages of
+// radix <= 16: the forward plan is [R0, 16, ..., 16] and the inverse plan is
+// the reverse [16, ..., 16, R0] with R0 = N / 16^(S-1).  Each of the T = N/16
+// threads of one transform holds 16 complex values in registers; at a stage of
+// radix R it performs 16/R butterflies j = t + b*T.  Stage s (Ns = product of
+// the radices before it) reads x[j + r*N/R], applies twiddles w^(r*k)
+// (k = j mod Ns, w = e^{dir 2 pi i/(Ns R)}), does a radix-R DFT and writes
+// y[(j/Ns)*Ns*R + k + r*Ns] (Stockham autosort, natural-order output).
+//
+// Register layout contract: before the first stage and after the last stage,
+// register v[b*R + r] of thread t holds element t + b*T + r*N/R where R is the
+// radix of that stage.  Because the inverse plan starts with the radix the
+// forward plan ends with, a forward transform, an element-wise multiply and an
+// inverse transform chain in registers without an LDS exchange in between.
+//
+// Twiddles come from a table tw[m] = e^{-2 pi i m/N} (m < N) built in FP64 on
+// the host and rounded to float; the inverse uses the conjugate.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace atsim {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by e^{DIR i pi/2}: DIR=-1 -> -i, DIR=+1 -> +i
+template <int DIR>
+__device__ __forceinline__ float2 mul_qi(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// cos / sin of pi*m/8, m = 0..9
+__device__ __forceinline__ float2 w16(int m, int dir) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, H = 0.70710678118654752f;
+    float c, s;
+    switch (m) {
+        case 0: c = 1.f; s = 0.f; break;
+        case 1: c = C1; s = S1; break;
+        case 2: c = H; s = H; break;
+        case 3: c = S1; s = C1; break;
+        case 4: c = 0.f; s = 1.f; break;
+        case 6: c = -H; s = H; break;
+        case 9: c = -C1; s = -S1; break;
+        default: c = 1.f; s = 0.f; break;
+    }
+    return make_float2(c, dir < 0 ? -s : s);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    float2 t2 = cadd(a1, a3), t3 = mul_qi<DIR>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+// 8 = 2 x 4: n = n1 + 2 n2, k = 4 k1 + k2
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* v) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    constexpr float H = 0.70710678118654752f;
+    // o_k *= e^{DIR 2 pi i k / 8}
+    o1 = cmul(o1, make_float2(H, DIR < 0 ? -H : H));
+    o2 = mul_qi<DIR>(o2);
+    o3 = cmul(o3, make_float2(-H, DIR < 0 ? -H : H));
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+// 16 = 4 x 4: n = n1 + 4 n2, k = 4 k1 + k2
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* v) {
+    float2 a[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++) {
+        a[n1][0] = v[n1]; a[n1][1] = v[n1 + 4]; a[n1][2] = v[n1 + 8]; a[n1][3] = v[n1 + 12];
+        dft4<DIR>(a[n1][0], a[n1][1], a[n1][2], a[n1][3]);
+    }
+#pragma unroll
+    for (int n1 = 1; n1 < 4; n1++)
+#pragma unroll
+        for (int k2 = 1; k2 < 4; k2++) a[n1][k2] = cmul(a[n1][k2], w16(n1 * k2, DIR));
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) {
+        float2 b0 = a[0][k2], b1 = a[1][k2], b2 = a[2][k2], b3 = a[3][k2];
+        dft4<DIR>(b0, b1, b2, b3);
+        v[k2] = b0; v[k2 + 4] = b1; v[k2 + 8] = b2; v[k2 + 12] = b3;
+    }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dftR(float2* v) {
+    if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
+    else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 8) dft8<DIR>(v);
+    else dft16<DIR>(v);
+}
+
+// ---------------------------------------------------------------- plans
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+template <int N>
+struct Plan {
+    static constexpr int LOG = ilog2c(N);
+    static constexpr int S = (LOG + 3) / 4;                    // stages
+    static constexpr int R0 = N >> (4 * (S - 1));              // first forward radix
+    static constexpr int T = N / 16;                           // threads per transform
+    static_assert((1 << LOG) == N && N >= 16 && N <= 8192, "N must be a power of two in [16, 8192]");
+    // radix of stage s in direction DIR (forward [R0,16..], inverse [..16,R0])
+    static constexpr int radix(int s, int dir) { return dir < 0 ? (s == 0 ? R0 : 16) : (s == S - 1 ? R0 : 16); }
+    static constexpr int ns(int s, int dir) {  // product of radices of earlier stages
+        int p = 1;
+        for (int q = 0; q < s; q++) p *= radix(q, dir);
+        return p;
+    }
+    static constexpr int first(int dir) { return radix(0, dir); }
+    static constexpr int last(int dir) { return radix(S - 1, dir); }
+    static constexpr int LDS = N + N / 16;                     // padded LDS elements per transform
+};
+
+// LDS addressing: element idx of transform lane `c` in a group of C interleaved
+// transforms; one pad slot every 16 elements.
+template <int C>
+__device__ __forceinline__ int lds_addr(int idx, int c) { return (idx + (idx >> 4)) * C + c; }
+
+template <int N, int DIR>
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
+    float2 w = tw[m];
+    return DIR < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// one stage's twiddle + butterfly on registers
+template <int N, int R, int NS, int DIR>
+__device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw) {
+    constexpr int T = N / 16;
+#pragma unroll
+    for (int b = 0; b < 16 / R; b++) {
+        if constexpr (NS > 1) {
+            const int j = t + b * T;
+            const int k = j % NS;
+            constexpr int STEP = N / (NS * R);
+#pragma unroll
+            for (int r = 1; r < R; r++) v[b * R + r] = cmul(v[b * R + r], twiddle<N, DIR>(tw, r * k * STEP));
+        }
+        dftR<R, DIR>(v + b * R);
+    }
+}
+
+template <int N, int R, int NS, int C>
+__device__ __forceinline__ void stage_store(const float2* v, float2* lds, int t, int c) {
+    constexpr int T = N / 16;
+#pragma unroll
+    for (int b = 0; b < 16 / R; b++) {
+        const int j = t + b * T;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) lds[lds_addr<C>(base + r * NS, c)] = v[b * R + r];
+    }
+}
+
+template <int N, int R, int C>
+__device__ __forceinline__ void stage_load(float2* v, const float2* lds, int t, int c) {
+    constexpr int T = N / 16;
+#pragma unroll
+    for (int b = 0; b < 16 / R; b++)
+#pragma unroll
+        for (int r = 0; r < R; r++) v[b * R + r] = lds[lds_addr<C>(t + b * T + r * (N / R), c)];
+}
+
+// Full transform on registers.  All threads of the workgroup must call it
+// (it contains __syncthreads); `lds` is this transform group's region.
+template <int N, int DIR, int C>
+__device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, int c, const float2* __restrict__ tw) {
+    using P = Plan<N>;
+    stage_compute<N, P::radix(0, DIR), P::ns(0, DIR), DIR>(v, t, tw);
+    if constexpr (P::S > 1) {
+        __syncthreads();
+        stage_store<N, P::radix(0, DIR), P::ns(0, DIR), C>(v, lds, t, c);
+        __syncthreads();
+        stage_load<N, P::radix(1, DIR), C>(v, lds, t, c);
+        stage_compute<N, P::radix(1, DIR), P::ns(1, DIR), DIR>(v, t, tw);
+    }
+    if constexpr (P::S > 2) {
+        __syncthreads();
+        stage_store<N, P::radix(1, DIR), P::ns(1, DIR), C>(v, lds, t, c);
+        __syncthreads();
+        stage_load<N, P::radix(2, DIR), C>(v, lds, t, c);
+        stage_compute<N, P::radix(2, DIR), P::ns(2, DIR), DIR>(v, t, tw);
+    }
+    if constexpr (P::S > 3) {
+        __syncthreads();
+        stage_store<N, P::radix(2, DIR), P::ns(2, DIR), C>(v, lds, t, c);
+        __syncthreads();
+        stage_load<N, P::radix(3, DIR), C>(v, lds, t, c);
+        stage_compute<N, P::radix(3, DIR), P::ns(3, DIR), DIR>(v, t, tw);
+    }
+}
+
+// index held by register slot q = b*R + r in the layout of a stage of radix R
+template <int N, int R>
+__device__ __forceinline__ int reg_index(int t, int q) {
+    return t + (q / R) * (N / 16) + (q % R) * (N / R);
+}
+
+}  // namespace atsim

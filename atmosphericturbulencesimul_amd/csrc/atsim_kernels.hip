@@ -1,0 +1,671 @@
+// atsim_kernels.hip -- CDNA4 (gfx950) kernels of the wavefront path.
+//
+//  k_row / k_col   LDS Stockham FFT passes along x / y.  OP_FRESNEL fuses
+//                  FFT -> transfer-function multiply -> IFFT in one pass
+//                  (the separable form of Fresnel_propagate_wavefront,
+//                  WFpropagate.c:68-152); k_row can also multiply the field by
+//                  the screen phase of a group of layers first, sampling the
+//                  master screens itself (AtmosphericTurbulence.c:2759-3412).
+//  k_permut        quadrant swap (fft.c:570-792).
+//  k_noise         Philox4x32-10 + Box-Muller complex Gaussian noise.
+//  k_synth_filter  von Karman spectral filter on the noise, written in the
+//                  quadrant-swapped layout (AtmosphericTurbulence.c:643-784).
+//  k_lag_sums, k_strfn  FP64 structure-function reductions
+//                  (AT.c:790-847, 2246-2270).
+//  k_extract       crop, amplitude, phase, geometric-phase unwrap
+//                  (AT.c:3418-3476), block sums of the science-lambda rule
+//                  (AT.c:3523-3625).
+//  k_block_fix / k_block_apply  the +-2 pi block consistency heuristic and
+//                  the final unwrap (AT.c:3627-4262).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "atsim_fft.hpp"
+#include "atsim_kernels.hpp"
+#include "atsim_launch.hpp"
+
+namespace atsim {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int wrap_m(int x, int M) {
+    x += (x < 0) ? M : 0;
+    x -= (x >= M) ? M : 0;
+    x -= (x >= M) ? M : 0;
+    return x;
+}
+
+// separable 4 x 4 interpolation of layer l's master around (x0 + i, y0 + j), FP64
+__device__ __forceinline__ double sample_layer(const float* __restrict__ m, int M, const LayerSample& p,
+                                               int i, int j) {
+    const int xb = p.x0 + i - 1, yb = p.y0 + j - 1;
+    const int x0 = wrap_m(xb, M), x1 = wrap_m(xb + 1, M), x2 = wrap_m(xb + 2, M), x3 = wrap_m(xb + 3, M);
+    double v = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const float* r = m + (size_t)wrap_m(yb + b, M) * M;
+        double h = p.wx[0] * (double)r[x0];
+        h = fma(p.wx[1], (double)r[x1], h);
+        h = fma(p.wx[2], (double)r[x2], h);
+        h = fma(p.wx[3], (double)r[x3], h);
+        v = fma(p.wy[b], h, v);
+    }
+    return v;
+}
+
+// e^{i ph}: argument reduced modulo 2 pi in FP64, sin/cos in FP32
+__device__ __forceinline__ float2 expi_d(double ph) {
+    const double inv2pi = 0.15915494309189533577;
+    const double twopi_hi = 6.28318530717958623200, twopi_lo = 2.44929359829470635445e-16;
+    double k = rint(ph * inv2pi);
+    double r = fma(-k, twopi_hi, ph);
+    r = fma(-k, twopi_lo, r);
+    float s, c;
+    sincosf((float)r, &s, &c);
+    return make_float2(c, s);
+}
+
+__device__ __forceinline__ double screen_phase(const ScreenArgs& sa, int d, int w, int i, int j) {
+    double ph = 0.0;
+    for (int q = 0; q < sa.lcount; q++) {
+        const int l = sa.lfirst + q;
+        const LayerSample p = sa.ls[d * sa.L + l];
+        const float* m = sa.masters + (size_t)l * sa.M * sa.M;
+        ph = fma(sa.s[w * sa.L + l], sample_layer(m, (int)sa.M, p, i, j), ph);
+    }
+    return ph;
+}
+
+// (long)(x + 1000.5) - 1000, the reference's rounding of unwrap counts
+__device__ __forceinline__ double unwrap_count(double x) { return (double)((long long)(x + 1000.5) - 1000); }
+
+// ------------------------------------------------------------------ configs
+template <int N>
+struct RowCfg {
+    static constexpr int T = N / 16;
+    static constexpr int ROWS0 = 256 / T > 0 ? 256 / T : 1;
+    static constexpr int ROWS = ROWS0 < N ? ROWS0 : N;
+    static constexpr int THREADS = ROWS * T;
+};
+template <int N>
+struct ColCfg {
+    static constexpr int C0 = N <= 1024 ? 16 : (N == 2048 ? 8 : (N == 4096 ? 4 : 2));
+    static constexpr int C = C0 < N ? C0 : N;
+    static constexpr int THREADS = C * (N / 16);
+};
+
+// ------------------------------------------------------------------ passes
+template <int N, int OP, int DIR, bool GEN>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+k_row(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw,
+      ScreenArgs sa) {
+    using P = Plan<N>;
+    constexpr int T = P::T, ROWS = RowCfg<N>::ROWS;
+    constexpr int RI = (OP == OP_FRESNEL) ? P::first(-1) : P::first(DIR);
+    constexpr int RO = (OP == OP_FRESNEL) ? P::last(+1) : P::last(DIR);
+    __shared__ float2 lds[ROWS * P::LDS];
+    const int lr = threadIdx.x / T, t = threadIdx.x - lr * T;
+    const int row = blockIdx.x * ROWS + lr;
+    const int f = blockIdx.y;
+    float2* rowp = data + ((size_t)f * N + row) * N;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = GEN ? make_float2(1.f, 0.f) : rowp[reg_index<N, RI>(t, q)];
+    if (sa.lcount > 0) {
+        const int w = f % sa.W, d = f / sa.W;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const double ph = screen_phase(sa, d, w, reg_index<N, RI>(t, q), row);
+            v[q] = cmul(v[q], expi_d(ph));
+        }
+    }
+    float2* L = lds + lr * P::LDS;
+    if constexpr (OP == OP_FRESNEL) {
+        fft_regs<N, -1, 1>(v, L, t, 0, tw);
+        const float2* h = tf + (size_t)(f % W) * N;
+        constexpr int RL = P::last(-1);
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t, q)]);
+        fft_regs<N, +1, 1>(v, L, t, 0, tw);
+    } else {
+        fft_regs<N, DIR, 1>(v, L, t, 0, tw);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) rowp[reg_index<N, RO>(t, q)] = v[q];
+}
+
+template <int N, int OP, int DIR>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw) {
+    using P = Plan<N>;
+    constexpr int C = ColCfg<N>::C;
+    constexpr int RI = (OP == OP_FRESNEL) ? P::first(-1) : P::first(DIR);
+    constexpr int RO = (OP == OP_FRESNEL) ? P::last(+1) : P::last(DIR);
+    __shared__ float2 lds[C * P::LDS];
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int col = blockIdx.x * C + c;
+    const int f = blockIdx.y;
+    float2* base = data + (size_t)f * N * N + col;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = base[(size_t)reg_index<N, RI>(t, q) * N];
+    if constexpr (OP == OP_FRESNEL) {
+        fft_regs<N, -1, C>(v, lds, t, c, tw);
+        const float2* h = tf + (size_t)(f % W) * N;
+        constexpr int RL = P::last(-1);
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t, q)]);
+        fft_regs<N, +1, C>(v, lds, t, c, tw);
+    } else {
+        fft_regs<N, DIR, C>(v, lds, t, c, tw);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) base[(size_t)reg_index<N, RO>(t, q) * N] = v[q];
+}
+
+template <int N>
+static hipError_t row_dispatch(int op, int dir, bool gen, float2* data, long F, const float2* tf, int W,
+                               const float2* tw, const ScreenArgs& sa, hipStream_t st) {
+    dim3 grid(N / RowCfg<N>::ROWS, (unsigned)F), block(RowCfg<N>::THREADS);
+    if (op == OP_FRESNEL) {
+        if (gen) k_row<N, OP_FRESNEL, 0, true><<<grid, block, 0, st>>>(data, tf, W, tw, sa);
+        else k_row<N, OP_FRESNEL, 0, false><<<grid, block, 0, st>>>(data, tf, W, tw, sa);
+    } else if (dir < 0) {
+        k_row<N, OP_FFT, -1, false><<<grid, block, 0, st>>>(data, tf, W, tw, sa);
+    } else {
+        k_row<N, OP_FFT, +1, false><<<grid, block, 0, st>>>(data, tf, W, tw, sa);
+    }
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t col_dispatch(int op, int dir, float2* data, long F, const float2* tf, int W,
+                               const float2* tw, hipStream_t st) {
+    dim3 grid(N / ColCfg<N>::C, (unsigned)F), block(ColCfg<N>::THREADS);
+    if (op == OP_FRESNEL) k_col<N, OP_FRESNEL, 0><<<grid, block, 0, st>>>(data, tf, W, tw);
+    else if (dir < 0) k_col<N, OP_FFT, -1><<<grid, block, 0, st>>>(data, tf, W, tw);
+    else k_col<N, OP_FFT, +1><<<grid, block, 0, st>>>(data, tf, W, tw);
+    return hipGetLastError();
+}
+
+bool supported_n(long n) { return n >= 16 && n <= 8192 && (n & (n - 1)) == 0; }
+
+#define ATSIM_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, const float2* tf,
+                      const float2* tw, const ScreenArgs& sa, hipStream_t st) {
+    const int W = sa.W > 0 ? sa.W : 1;
+    switch (n) {
+#define X(NN) \
+    case NN: return row_dispatch<NN>(op, dir, gen, data, F, tf, W, tw, sa, st);
+        ATSIM_FOR_EACH_N(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
+                      const float2* tw, hipStream_t st) {
+    switch (n) {
+#define X(NN) \
+    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, st);
+        ATSIM_FOR_EACH_N(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------------ permut
+__global__ void k_permut(float2* __restrict__ d, long n0, long n1, long nbatch) {
+    const long half = n0 * (n1 / 2);
+    const long total = half * nbatch;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long b = e / half, r = e - b * half;
+        const long jj = r / n0, ii = r - jj * n0;
+        float2* img = d + b * n0 * n1;
+        const long a = jj * n0 + ii;
+        const long p = (jj + n1 / 2) * n0 + (ii + n0 / 2) % n0;
+        float2 t = img[a];
+        img[a] = img[p];
+        img[p] = t;
+    }
+}
+
+hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st) {
+    long total = n0 * (n1 / 2) * nbatch;
+    int blocks = (int)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+    if (blocks < 1) blocks = 1;
+    k_permut<<<blocks, 256, 0, st>>>(data, n0, n1, nbatch);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ noise
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__global__ void k_noise(float2* __restrict__ out, long n, uint32_t k0, uint32_t k1) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        uint32_t c[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), 0x0A75u, 0u};
+        philox10(c, k0, k1);
+        const uint64_t a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+        const uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+        const double u1 = ((double)a + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = (double)b * (1.0 / 9007199254740992.0);
+        const double r = sqrt(-2.0 * log(u1));
+        double s, co;
+        sincos(2.0 * M_PI * u2, &s, &co);
+        out[e] = make_float2((float)(r * co), (float)(r * s));
+    }
+}
+
+hipError_t launch_noise(float2* out, long n, uint64_t seed, uint32_t stream, hipStream_t st) {
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    if (blocks < 1) blocks = 1;
+    k_noise<<<blocks, 256, 0, st>>>(out, n, (uint32_t)seed, (uint32_t)(seed >> 32) ^ stream);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ synthesis
+// storage index p holds centred index (p + s/2) mod s after the quadrant swap
+__global__ void k_synth_filter(const float2* __restrict__ noise, float2* __restrict__ out, long size,
+                               double f0, double fi, double rlim) {
+    const long n = size * size, h = size / 2;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long py = e / size, px = e - py * size;
+        const long cx = (px + h) % size, cy = (py + h) % size;
+        const double dx = (double)(cx - h), dy = (double)(cy - h);
+        const double r2 = dx * dx + dy * dy;
+        double A = 0.0;
+        const double tmpd = sqrt(r2 + f0 * f0);
+        const bool inside = rlim > 0.0 && sqrt(r2) < rlim;
+        if (!inside && tmpd > 0.0) A = sqrt(exp(-r2 / fi / fi)) / pow(tmpd, 11.0 / 6.0);
+        if (cx == h && cy == h) A = 0.0;
+        const float2 z = noise[cy * size + cx];
+        out[e] = make_float2((float)(A * (double)z.x), (float)(A * (double)z.y));
+    }
+}
+
+hipError_t launch_synth_filter(const float2* noise, float2* out, long size, double f0, double fi, double rlim,
+                               hipStream_t st) {
+    long n = size * size;
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    k_synth_filter<<<blocks, 256, 0, st>>>(noise, out, size, f0, fi, rlim);
+    return hipGetLastError();
+}
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double* __restrict__ out) {
+    __shared__ double red[NV][256 / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[k][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[threadIdx.x][q];
+        out[threadIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_lag_sums(const float2* __restrict__ c, long size, double* partial) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const long n = size * size;
+    const int la[4] = {1, 1, 2, 2}, lb[4] = {1, 2, 1, 2};
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long y = e / size, x = e - y * size;
+        const float2 p = c[e];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long x2 = (x + la[q]) % size, y2 = (y + lb[q]) % size;
+            const float2 o = c[y2 * size + x2];
+            const double dr = (double)p.x - (double)o.x, di = (double)p.y - (double)o.y;
+            acc[q] = fma(dr, dr, acc[q]);
+            acc[4 + q] = fma(di, di, acc[4 + q]);
+        }
+    }
+    block_reduce_store<8>(acc, partial + (size_t)blockIdx.x * 8);
+}
+
+hipError_t launch_lag_sums(const float2* c, long size, double* partial, int nblocks, hipStream_t st) {
+    k_lag_sums<<<nblocks, 256, 0, st>>>(c, size, partial);
+    return hipGetLastError();
+}
+
+__global__ void k_split_scale(const float2* __restrict__ c, float* __restrict__ o1, float* __restrict__ o2,
+                              long n, double s1, double s2) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const float2 p = c[e];
+        o1[e] = (float)((double)p.x * s1);
+        if (o2) o2[e] = (float)((double)p.y * s2);
+    }
+}
+
+hipError_t launch_split_scale(const float2* c, float* o1, float* o2, long n, double s1, double s2,
+                              hipStream_t st) {
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    k_split_scale<<<blocks, 256, 0, st>>>(c, o1, o2, n, s1, s2);
+    return hipGetLastError();
+}
+
+// sum of (s(x, y) - s(x + a, y + b))^2; circular, or over x < size - a
+// (non circular in x, as the r0 measure) when circular == 0 (b must be 0)
+__global__ void __launch_bounds__(256) k_strfn(const float* __restrict__ s, long size, long a, long b,
+                                              int circular, double* partial) {
+    double acc[1] = {0.0};
+    const long nx = circular ? size : size - a;
+    const long n = nx * size;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long y = e / nx, x = e - y * nx;
+        const long x2 = (x + a) % size, y2 = ((y + b) % size + size) % size;
+        const double d = (double)s[y * size + x] - (double)s[y2 * size + x2];
+        acc[0] = fma(d, d, acc[0]);
+    }
+    block_reduce_store<1>(acc, partial + blockIdx.x);
+}
+
+hipError_t launch_strfn(const float* s, long size, long a, long b, int circular, double* partial, int nblocks,
+                        hipStream_t st) {
+    k_strfn<<<nblocks, 256, 0, st>>>(s, size, a, b, circular, partial);
+    return hipGetLastError();
+}
+
+__global__ void k_scale(float* __restrict__ s, long n, double scale) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        s[e] = (float)((double)s[e] * scale);
+}
+
+hipError_t launch_scale(float* s, long n, double scale, hipStream_t st) {
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    if (blocks < 1) blocks = 1;
+    k_scale<<<blocks, 256, 0, st>>>(s, n, scale);
+    return hipGetLastError();
+}
+
+__global__ void k_cmul(float2* __restrict__ a, const float2* __restrict__ b, long n, long nbatch) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n * nbatch; e += (long)gridDim.x * blockDim.x) {
+        const float2 x = a[e], y = b[e % n];
+        // products in FP64 as Fresnel_propagate_wavefront1 (WFpropagate.c:302-310)
+        a[e] = make_float2((float)((double)x.x * y.x - (double)x.y * y.y),
+                           (float)((double)x.x * y.y + (double)x.y * y.x));
+    }
+}
+
+hipError_t launch_cmul(float2* a, const float2* b, long n, long nbatch, hipStream_t st) {
+    long tot = n * nbatch;
+    int blocks = (int)((tot + 255) / 256 < 16384 ? (tot + 255) / 256 : 16384);
+    if (blocks < 1) blocks = 1;
+    k_cmul<<<blocks, 256, 0, st>>>(a, b, n, nbatch);
+    return hipGetLastError();
+}
+
+__global__ void k_amp_pha(const float2* __restrict__ e, float* __restrict__ amp, float* __restrict__ pha, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double re = e[i].x, im = e[i].y;
+        amp[i] = (float)sqrt(re * re + im * im);
+        pha[i] = (float)atan2(im, re);
+    }
+}
+
+hipError_t launch_amp_pha(const float2* e, float* amp, float* pha, long n, hipStream_t st) {
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    if (blocks < 1) blocks = 1;
+    k_amp_pha<<<blocks, 256, 0, st>>>(e, amp, pha, n);
+    return hipGetLastError();
+}
+
+// Init_Fresnel_propagate_wavefront (WFpropagate.c:166-258), written directly
+// at the quadrant-swapped position (WFpropagate.c:258).
+__global__ void k_tf_init(float2* __restrict__ C, long size, double coeff, double fpmaskrad) {
+    const long n = size * size;
+    const double co1 = (double)size * (double)size;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const long jj = e / size, ii = e - jj * size;
+        double Re = 0.0, Im = 0.0;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                const double x = 1.0 * ii + 1.0 * (0.5 + i - 2) / 4 - (double)(size / 2);
+                const double y = 1.0 * jj + 1.0 * (0.5 + j - 2) / 4 - (double)(size / 2);
+                const double sq = x * x + y * y;
+                if (sqrt(sq) < fpmaskrad) {
+                    double s, c;
+                    sincos(-coeff * sq, &s, &c);
+                    Re += c;
+                    Im += s;
+                }
+            }
+        double Amp = sqrt(Re * Re + Im * Im) / 16.0;
+        const double Pha = atan2(Im, Re);
+        Amp = (Amp > 0.3) ? 1.0 : 0.0;
+        const long dx = (ii + size / 2) % size, dy = (jj + size / 2) % size;
+        C[dy * size + dx] = make_float2((float)(Amp * cos(Pha) / co1), (float)(Amp * sin(Pha) / co1));
+    }
+}
+
+hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, hipStream_t st) {
+    long n = size * size;
+    int blocks = (int)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+    k_tf_init<<<blocks, 256, 0, st>>>(C, size, coeff, fpmaskrad);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ extraction
+// One workgroup per (row, direction, wavelength chunk of <= 8).  Every layer
+// is sampled once per pixel (FP64) and shared by the chunk's wavelengths.
+template <bool FULL>
+__global__ void __launch_bounds__(256) k_extract(ExtractArgs a) {
+    extern __shared__ double bs[];  // [8][2][xs]
+    const int row = blockIdx.x, d = blockIdx.y;
+    a.w_begin = blockIdx.z * 8;
+    const int o = (a.N - a.Nout) / 2;
+    const int jf = FULL ? row : row + o;
+    const int jj = jf - o;
+    const bool rowin = jj >= 0 && jj < a.Nout;
+    const int ncols = FULL ? a.N : a.Nout;
+    const int Wc = min(8, a.W - a.w_begin);
+    const int xs = a.xs;
+    const int jj2 = rowin ? (int)((double)jj / a.Nout * xs) : -1;
+    const bool blockrow = rowin && jj2 >= 0 && jj2 < xs;
+    const int L = a.all.L;
+    for (int k = threadIdx.x; k < 16 * xs; k += blockDim.x) bs[k] = 0.0;
+    __syncthreads();
+    for (int cc = threadIdx.x; cc < ncols; cc += blockDim.x) {
+        const int ifl = FULL ? cc : cc + o;
+        const int ii = ifl - o;
+        const bool colin = ii >= 0 && ii < a.Nout;
+        double PHI[8], PL[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
+        for (int l = 0; l < L; l++) {
+            const LayerSample p = a.all.ls[d * L + l];
+            const double v = sample_layer(a.all.masters + (size_t)l * a.all.M * a.all.M, (int)a.all.M, p, ifl, jf);
+            const bool inlast = l >= a.last_first && l < a.last_first + a.last_count;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                if (w < Wc) {
+                    const double sv = a.all.s[(a.w_begin + w) * L + l] * v;
+                    PHI[w] += sv;
+                    if (inlast) PL[w] += sv;
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            if (w >= Wc) continue;
+            const int f = d * a.W + a.w_begin + w;
+            float2 E = a.fields ? a.fields[((size_t)f * a.N + jf) * a.N + ifl] : make_float2(1.f, 0.f);
+            if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
+            if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
+            if (!(rowin && colin)) continue;
+            const size_t oi = ((size_t)f * a.Nout + jj) * a.Nout + ii;
+            if (!a.amplitude) {
+                a.pha[oi] = (float)PHI[w];
+                a.amp[oi] = 1.f;
+                continue;
+            }
+            const double re = E.x, im = E.y;
+            a.amp[oi] = (float)sqrt(re * re + im * im);
+            const double ph = (double)atan2f(E.y, E.x);
+            if (a.unwrap_mode[a.w_begin + w] == 0) {
+                a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] / 2.0 / M_PI));
+            } else {
+                a.pha[oi] = (float)ph;
+                if (a.dsave) a.dsave[oi] = PHI[w] - ph;
+                const int ii2 = (int)((double)ii / a.Nout * xs);
+                if (blockrow && ii2 < xs) {
+                    const float2 cs = expi_d(PHI[w] - ph);
+                    atomicAdd(&bs[(w * 2 + 0) * xs + ii2], (double)cs.x);
+                    atomicAdd(&bs[(w * 2 + 1) * xs + ii2], (double)cs.y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!blockrow || !a.amplitude || a.bsum == nullptr) return;
+    const int nb = xs * xs;
+    for (int k = threadIdx.x; k < Wc * xs; k += blockDim.x) {
+        const int w = k / xs, ii2 = k - w * xs;
+        if (a.unwrap_mode[a.w_begin + w] == 0) continue;
+        const int f = d * a.W + a.w_begin + w;
+        double* dst = a.bsum + ((size_t)f * nb + (size_t)jj2 * xs + ii2) * 2;
+        atomicAdd(dst, bs[(w * 2 + 0) * xs + ii2]);
+        atomicAdd(dst + 1, bs[(w * 2 + 1) * xs + ii2]);
+    }
+}
+
+hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st) {
+    dim3 grid(full ? a.N : a.Nout, a.D, (a.W + 7) / 8), block(256);
+    size_t sh = (size_t)16 * (a.xs > 0 ? a.xs : 1) * sizeof(double);
+    if (full) k_extract<true><<<grid, block, sh, st>>>(a);
+    else k_extract<false><<<grid, block, sh, st>>>(a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ block rule
+// The twelve neighbour relations of the block fix-up (AT.c:3667-3903): block
+// (ii2+dx1, jj2+dy1) vs (ii2+dx2, jj2+dy2) for ii2 < xs-xm, jj2 < ys-ym.
+__constant__ int c_pairs[12][6] = {
+    {0, 0, 1, 0, 1, 0}, {0, 0, 0, 1, 0, 1}, {0, 0, 1, 1, 1, 1}, {0, 1, 1, 0, 1, 1},
+    {0, 0, 2, 0, 2, 0}, {0, 2, 0, 0, 0, 2}, {1, 2, 0, 0, 1, 2}, {0, 2, 1, 0, 1, 2},
+    {2, 2, 0, 0, 2, 2}, {0, 2, 2, 0, 2, 2}, {0, 1, 2, 0, 2, 1}, {2, 1, 0, 0, 2, 1}};
+
+__global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ bsum, double* __restrict__ bin,
+                                                    double* __restrict__ ch, const int* __restrict__ flist, int xs) {
+    __shared__ double redd[16];
+    __shared__ int redi[16];
+    const int f = flist[blockIdx.x];
+    const int nb = xs * xs;
+    const double* S = bsum + (size_t)f * nb * 2;
+    double* B = bin + (size_t)f * nb;
+    double* Cc = ch + (size_t)f * nb;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) B[b] = atan2(S[2 * b + 1], S[2 * b]);
+    __syncthreads();
+    int chiter = 0, chcnt0cnt = 0;
+    double plim = 1.0;
+    while (plim > 0.51 && chiter < 1000 && chcnt0cnt < 5) {
+        chiter++;
+        double mx = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int i2 = b % xs, j2 = b / xs;
+            const double pb = B[b];
+            double c = 0.0;
+            for (int q = 0; q < 12; q++) {
+                const int dx1 = c_pairs[q][0], dy1 = c_pairs[q][1], dx2 = c_pairs[q][2], dy2 = c_pairs[q][3];
+                const int xm = c_pairs[q][4], ym = c_pairs[q][5];
+                const double cq = q < 4 ? 0.2 : 0.15;
+                int x = i2 - dx1, y = j2 - dy1;  // b as the first block of the pair
+                if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
+                    const double pv2 = B[(y + dy2) * xs + x + dx2];
+                    if (pv2 > pb + M_PI) c += cq;
+                    if (pb > pv2 + M_PI) c -= cq;
+                }
+                x = i2 - dx2; y = j2 - dy2;  // b as the second block
+                if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
+                    const double pv1 = B[(y + dy1) * xs + x + dx1];
+                    if (pb > pv1 + M_PI) c -= cq;
+                    if (pv1 > pb + M_PI) c += cq;
+                }
+            }
+            Cc[b] = c;
+            mx = fmax(mx, fabs(c));
+        }
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+        if (lane == 0) redd[wv] = mx;
+        __syncthreads();
+        double m = 0.0;
+        for (int q = 0; q < nw; q++) m = fmax(m, redd[q]);
+        plim = m - 0.001;
+        if (plim < 0.5) plim = 0.5;
+        int cnt = 0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int i2 = b % xs, j2 = b / xs;
+            const bool interior = i2 > 1 && j2 > 1 && i2 < xs - 2 && j2 < xs - 2;
+            const double c = Cc[b];
+            if (c > plim) { if (interior) cnt++; B[b] += 2.0 * M_PI; }
+            if (c < -plim) { if (interior) cnt++; B[b] -= 2.0 * M_PI; }
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+        __syncthreads();
+        if (lane == 0) redi[wv] = cnt;
+        __syncthreads();
+        int chcnt = 0;
+        for (int q = 0; q < nw; q++) chcnt += redi[q];
+        if (chcnt == 0) chcnt0cnt++;
+        else chcnt0cnt = 0;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* field_list, int nfields, int xs,
+                            hipStream_t st) {
+    if (nfields <= 0 || xs <= 0) return hipSuccess;
+    k_block_fix<<<nfields, 1024, 0, st>>>(bsum, bin, ch, field_list, xs);
+    return hipGetLastError();
+}
+
+__global__ void k_block_apply(float* __restrict__ pha, const double* __restrict__ dsave,
+                              const double* __restrict__ bin, const int* __restrict__ flist, int Nout, int xs) {
+    const int f = flist[blockIdx.y];
+    const long no = (long)Nout * Nout;
+    const int nb = xs * xs;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < no; e += (long)gridDim.x * blockDim.x) {
+        const int jj = (int)(e / Nout), ii = (int)(e - (long)jj * Nout);
+        double peak = 0.0;
+        if (xs > 0) {
+            const int ii2 = (int)((double)ii / Nout * xs), jj2 = (int)((double)jj / Nout * xs);
+            if (ii2 < xs && jj2 < xs) peak = bin[(size_t)f * nb + jj2 * xs + ii2];
+        }
+        const size_t oi = (size_t)f * no + e;
+        const double p = pha[oi];
+        pha[oi] = (float)(p + 2.0 * M_PI * unwrap_count((dsave[oi] - peak) / 2.0 / M_PI));
+    }
+}
+
+hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
+                              int nfields, int Nout, int xs, hipStream_t st) {
+    if (nfields <= 0) return hipSuccess;
+    long no = (long)Nout * Nout;
+    int bx = (int)((no + 255) / 256 < 4096 ? (no + 255) / 256 : 4096);
+    dim3 grid(bx, nfields);
+    k_block_apply<<<grid, 256, 0, st>>>(pha, dsave, bin, field_list, Nout, xs);
+    return hipGetLastError();
+}
+
+}  // namespace atsim

@@ -1,0 +1,47 @@
+// atsim_kernels.hpp -- kernel argument structures shared by the kernels and
+// the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace atsim {
+
+// Per (direction, layer) sampling parameters of one frame: integer base
+// position (x0, y0) in [0, M) and separable interpolation weights for taps at
+// offsets -1, 0, +1, +2.  Built on the host in FP64 from the cumulative layer
+// offsets (AtmosphericTurbulence.c:2700-2755, samplers 2761-2922).
+struct LayerSample {
+    int x0, y0;
+    int pad0, pad1;
+    double wx[4];
+    double wy[4];
+};
+
+// Screen-phase source: phase(i, j) of field f = sum over layers l in
+// [lfirst, lfirst + lcount) of s[w][l] * interp_l(i, j), w = f % W,
+// direction = f / W.
+struct ScreenArgs {
+    const float* masters;    // L x M x M
+    const LayerSample* ls;   // D x L
+    const double* s;         // W x L
+    long M;
+    int L, W, lfirst, lcount;
+};
+
+struct ExtractArgs {
+    const float2* fields;    // [F][N][N] final-but-one field, or null (field == 1)
+    float2* field_out;       // [F][N][N] or null
+    float* pha;              // [F][Nout][Nout]
+    float* amp;              // [F][Nout][Nout]
+    double* dsave;           // [F][Nout][Nout] PHI - atan2 for block-rule fields (or null)
+    double* bsum;            // [F][nb][2] block sums of cos/sin(PHI - atan2)
+    const int* unwrap_mode;  // [W]
+    ScreenArgs all;          // every layer: the geometric phase PHI
+    int last_first, last_count;  // super-layer group whose screen is applied here
+    int N, Nout, W, D;
+    int amplitude;           // 0: phase only (pha = PHI, amp = 1)
+    int xs;                  // Nout / 20 blocks per side
+    int w_begin;             // first field of this chunk of <= 8 wavelengths
+};
+
+}  // namespace atsim

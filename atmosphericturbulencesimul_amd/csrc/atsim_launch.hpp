@@ -1,0 +1,46 @@
+// atsim_launch.hpp -- host-side launchers of the HIP kernels (atsim_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "atsim_kernels.hpp"
+
+namespace atsim {
+
+enum { OP_FFT = 0, OP_FRESNEL = 1 };
+
+bool supported_n(long n);
+
+// Row pass over F fields of n x n: optional screen multiply (sa.lcount > 0)
+// before the transform; op OP_FRESNEL = FFT_x, x h(kx) (table tf + (f % W) n),
+// IFFT_x; OP_FFT = one transform of sign dir.  gen: the input field is 1
+// (nothing is read).
+hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, const float2* tf,
+                      const float2* tw, const ScreenArgs& sa, hipStream_t st);
+// Column pass: same transforms along y; no screen.
+hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
+                      const float2* tw, hipStream_t st);
+
+hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st);
+hipError_t launch_noise(float2* out, long n, uint64_t seed, uint32_t stream, hipStream_t st);
+hipError_t launch_synth_filter(const float2* noise, float2* out, long size, double f0, double fi,
+                               double rlim, hipStream_t st);
+// partial sums of squared differences at lags (1,1),(1,2),(2,1),(2,2) of Re
+// and Im: partial[block][8]
+hipError_t launch_lag_sums(const float2* c, long size, double* partial, int nblocks, hipStream_t st);
+hipError_t launch_split_scale(const float2* c, float* o1, float* o2, long n, double s1, double s2,
+                              hipStream_t st);
+hipError_t launch_strfn(const float* s, long size, long a, long b, int circular, double* partial,
+                        int nblocks, hipStream_t st);
+hipError_t launch_scale(float* s, long n, double scale, hipStream_t st);
+hipError_t launch_cmul(float2* a, const float2* b, long n, long nbatch, hipStream_t st);
+hipError_t launch_amp_pha(const float2* e, float* amp, float* pha, long n, hipStream_t st);
+hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, hipStream_t st);
+
+hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
+hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* field_list, int nfields,
+                            int xs, hipStream_t st);
+hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
+                              int nfields, int Nout, int xs, hipStream_t st);
+
+}  // namespace atsim

@@ -1,0 +1,154 @@
+/*
+ * atsim.h -- C ABI of the MI355X wavefront library (libatsim.so).
+ *
+ * Plain C types only: device buffers are `void *` device pointers (HIP
+ * allocations on the context's device), sizes are `long`, status is `int`
+ * (0 = success, < 0 = error; the message is returned by atsim_error()).
+ * Nothing here exits the process (the reference's C functions print and
+ * exit on fatal errors; the registry-level compat layer in atsim_compat.h
+ * keeps that behaviour on top of these calls).
+ *
+ * Complex data are interleaved (re, im) float32 pairs, row-major with x
+ * fastest, exactly the reference's complex_float layout (CLIcore.h:130-140)
+ * so host images can be copied to the device byte for byte.
+ *
+ * Sizes: the HIP path handles square power-of-two grids 16 <= N <= 8192.
+ */
+#ifndef ATSIM_H
+#define ATSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct atsim_ctx atsim_ctx;
+typedef struct atsim_sim atsim_sim;
+
+#define ATSIM_OK 0
+#define ATSIM_EINVAL (-1)
+#define ATSIM_EHIP (-2)
+#define ATSIM_ENOMEM (-3)
+#define ATSIM_EUNSUPPORTED (-4)
+
+/* ---------------------------------------------------------------- context */
+/* One context = one device + one HIP stream.  Not thread safe; use one
+ * context per host thread. */
+int atsim_ctx_create(int device, atsim_ctx **out);
+void atsim_ctx_destroy(atsim_ctx *ctx);
+const char *atsim_error(const atsim_ctx *ctx);
+int atsim_sync(atsim_ctx *ctx);
+/* HIP stream (hipStream_t) all work of this context is queued on */
+void *atsim_stream(atsim_ctx *ctx);
+/* 1 if the library was built with code for the current device */
+int atsim_device_ok(atsim_ctx *ctx);
+
+/* ---------------------------------------------------------------- FFT */
+/* In-place unnormalised 2D c2c over nbatch contiguous n x n images.
+ * sign = -1 forward, +1 backward: the reference's do2dfft / do2dffti
+ * (src/fft/fft.c:1298-1315, FFTW plan over the swapped axes fft.c:1178-1190). */
+int atsim_fft2d(atsim_ctx *ctx, void *d_data, long n, long nbatch, int sign);
+/* Quadrant swap of nbatch complex n0 x n1 images, permut (fft.c:570-792);
+ * even sizes only on the device. */
+int atsim_permut(atsim_ctx *ctx, void *d_data, long n0, long n1, long nbatch);
+
+/* ---------------------------------------------------------------- Fresnel */
+/* Angular-spectrum Fresnel step of Fresnel_propagate_wavefront
+ * (src/WFpropagate/WFpropagate.c:68-152): out = IFFT[FFT[in] exp(-i pi z
+ * lambda (kx^2+ky^2)/(dx n)^2)/n^2], signed frequencies, on nbatch n x n
+ * fields.  d_in may equal d_out. */
+int atsim_fresnel(atsim_ctx *ctx, const void *d_in, void *d_out, long n, long nbatch,
+                  double pupil_scale, double z, double lambda);
+/* Init_Fresnel_propagate_wavefront (WFpropagate.c:166-261): fills the
+ * anti-aliased, permuted transfer function C (n x n complex, device). */
+int atsim_fresnel_tf_init(atsim_ctx *ctx, void *d_C, long n, double pupil_scale, double z,
+                          double lambda, double fpmaskrad);
+/* Fresnel_propagate_wavefront1 (WFpropagate.c:271-330): out = IFFT[FFT[in] C]. */
+int atsim_fresnel_tf_apply(atsim_ctx *ctx, const void *d_in, void *d_out, const void *d_C,
+                           long n, long nbatch);
+/* Fresnel_propagate_cube (WFpropagate.c:337-401): amplitude and phase cubes
+ * (n x n x nz float, device) for z_k = zstart + (zend - zstart) k / nz. */
+int atsim_fresnel_cube(atsim_ctx *ctx, const void *d_in, float *d_amp, float *d_pha, long n,
+                       double pupil_scale, double zstart, double zend, long nz, double lambda);
+
+/* ---------------------------------------------------------------- noise */
+/* Philox4x32-10 Gaussian complex noise (re, im ~ N(0,1) independent), element
+ * e of stream s uses counter (e_lo, e_hi, 0x0A75, 0) and key (seed_lo,
+ * seed_hi ^ s); Box-Muller in FP64.  Replaces the reference's clock-seeded
+ * make_rnd (src/image_gen/image_gen.c:1634-1740). */
+int atsim_noise_c64(atsim_ctx *ctx, void *d_out, long n, uint64_t seed, uint32_t stream);
+
+/* ---------------------------------------------------------------- screens */
+/* make_master_turbulence_screen (AtmosphericTurbulence.c:643-855) with the
+ * noise injected: C = A(r) z with the von Karman amplitude A, quadrant swap,
+ * forward FFT, out1 = Re/sqrt(C1), out2 = Im/sqrt(C2), where C is the 4-lag
+ * structure-function fit (AT.c:790-847).  d_noise: size^2 complex in the
+ * reference's centred layout.  d_out1/d_out2: size^2 float (d_out2 may be
+ * NULL).  C1/C2 (host) receive the fit coefficients.  rlim > 0 zeroes the
+ * spectrum inside rlim (the RLIM variable, AT.c:671-736). */
+int atsim_make_master(atsim_ctx *ctx, const void *d_noise, long size, double outerscale_px,
+                      double innerscale_px, double rlim, float *d_out1, float *d_out2,
+                      double *C1, double *C2);
+/* Structure function at lag (a, b), circular mean of squared differences,
+ * accumulated in FP64 (the quantity of fft_structure_function, info.c:1670). */
+int atsim_strfn(atsim_ctx *ctx, const float *d_screen, long size, long a, long b, double *out);
+/* r0 [pixels] from the x-lag dpix structure function, non circular
+ * (AT.c:2246-2270). */
+int atsim_r0_measure(atsim_ctx *ctx, const float *d_screen, long size, long dpix, double *r0);
+/* d_screen *= scale (float storage, AT.c:2274-2285 and 2339-2351). */
+int atsim_scale(atsim_ctx *ctx, float *d_screen, long n, double scale);
+
+/* ---------------------------------------------------------------- layers */
+/* Super-layer binning (AT.c:2067-2150).  alt must be ascending.  Writes
+ * slayer_alt[<=L] and super_layer_index[L]; returns the number of
+ * super-layers (>= 1) or < 0 on error. */
+long atsim_superlayers(const double *alt, const double *cn2, long L, double bin,
+                       double *slayer_alt, long *super_layer_index);
+
+/* ---------------------------------------------------------------- simulation */
+typedef struct {
+    long N;        /* WF_RAW_SIZE: propagation grid (power of 2) */
+    long Nout;     /* WFsize: centred output crop, Nout <= N */
+    long M;        /* MASTER_SIZE: master screen size, N <= M */
+    long L;        /* layers (ascending altitude) */
+    long W;        /* wavelengths (fields per direction) */
+    long D;        /* directions */
+    double dx;     /* PUPIL_SCALE [m/pixel] */
+    double dt;     /* WFTIME_STEP [s] */
+    double zangle; /* ZENITH_ANGLE [rad] */
+    int bicubic;   /* 1: Catmull-Rom (reference default, AT.c:1605); 0: bilinear */
+    int fresnel;   /* FRESNEL_PROPAGATION */
+    int amplitude; /* WAVEFRONT_AMPLITUDE */
+    const double *slayer_alt;       /* super-layer altitudes [m] */
+    const long *super_layer_index;  /* L */
+    const double *lambda;           /* W wavelengths [m] */
+    const double *scoeff;           /* W x L phase multipliers (1.0 for the reference lambda) */
+    const int *unwrap_mode;         /* W: 0 reference-lambda rule (AT.c:3456), 1 block rule (AT.c:3549-4262) */
+    const double *xpos0;            /* D x L initial x positions [pixels] */
+    const double *ypos0;            /* D x L initial y positions [pixels] */
+    const double *vx;               /* L [pixels/s] */
+    const double *vy;               /* L [pixels/s] */
+} atsim_sim_desc;
+
+int atsim_sim_create(atsim_ctx *ctx, const atsim_sim_desc *desc, atsim_sim **out);
+void atsim_sim_destroy(atsim_sim *sim);
+/* device pointer of layer l's M x M float master (write it directly, or use
+ * atsim_sim_set_master) */
+float *atsim_sim_master(atsim_sim *sim, long layer);
+/* copy an M x M float master (host or device pointer) into layer `layer` */
+int atsim_sim_set_master(atsim_sim *sim, long layer, const float *src);
+/* Advance every layer position by one time step (AT.c:2700-2755) and compute
+ * one frame for all D x W fields: d_pha / d_amp are device float arrays
+ * [D][W][Nout][Nout] (phase in rad, unwrapped; amplitude); d_field (nullable)
+ * receives the final complex fields [D][W][N][N]. */
+int atsim_sim_frame(atsim_sim *sim, float *d_pha, float *d_amp, void *d_field);
+/* current FP64 positions [D][L] (host arrays, nullable) */
+int atsim_sim_positions(atsim_sim *sim, double *xpos, double *ypos);
+/* number of Fresnel gaps per frame and of row/column passes per frame */
+long atsim_sim_gaps(atsim_sim *sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,97 @@
+"""GPU parity of the frame loop (layer sampling, screen phase, Fresnel gaps,
+extraction, both unwrap rules) against the FP64 oracle, through the C ABI.
+
+Bar: complex field and amplitude within relative L2 1e-5; phase within
+relative L2 1e-5 on pixels whose 2 pi decision agrees, with the number of
+disagreeing decisions reported and bounded."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+from tests.simcase import build_case, gpu_sim, oracle_sims
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _compare(c, osims, gs, nframes, tol=1e-5, max_mismatch_frac=1e-3):
+    field = torch.empty((c.D, c.W, c.N, c.N), dtype=torch.complex64, device="cuda")
+    worst = {"E": 0.0, "amp": 0.0, "pha": 0.0, "mism": 0}
+    for _ in range(nframes):
+        pha_g, amp_g = gs.frame(field)
+        fg = field.cpu().numpy()
+        pg, ag = pha_g.cpu().numpy(), amp_g.cpu().numpy()
+        for d, os_ in enumerate(osims):
+            E_o, _, pha_o, amp_o = os_.frame()
+            for w in range(c.W):
+                worst["E"] = max(worst["E"], rel_l2(fg[d, w], E_o[w]))
+                worst["amp"] = max(worst["amp"], rel_l2(ag[d, w], amp_o[w]))
+                diff = pg[d, w] - pha_o[w]
+                bad = np.abs(diff) > 1.0
+                worst["mism"] = max(worst["mism"], int(bad.sum()))
+                ok = ~bad
+                worst["pha"] = max(worst["pha"], rel_l2(pg[d, w][ok], pha_o[w][ok]))
+        xg, yg = gs.positions()
+        for d, os_ in enumerate(osims):
+            assert np.allclose(xg[d], os_.xpos, atol=0, rtol=1e-15)
+            assert np.allclose(yg[d], os_.ypos, atol=0, rtol=1e-15)
+    print("worst", worst)
+    assert worst["E"] < tol
+    assert worst["amp"] < tol
+    assert worst["pha"] < tol
+    assert worst["mism"] <= max_mismatch_frac * c.Nout * c.Nout
+    return worst
+
+
+def _cfg1_like(oracle, N=64, Nout=48, M=256, bicubic=True, **kw):
+    # 3 layers at 0/5/10 km, Cn2 0.5/0.3/0.2, wind 10/20/30 m/s at 0/90/200 deg
+    return build_case(oracle, N=N, Nout=Nout, M=M, alts=[0.0, 5000.0, 10000.0], cn2=[0.5, 0.3, 0.2],
+                      speeds=[10.0, 20.0, 30.0], dirs=np.radians([0.0, 90.0, 200.0]),
+                      lambdas=kw.pop("lambdas", [0.5e-6, 1.65e-6]), lam_ref=0.5e-6, dx=0.02, dt=0.001,
+                      r0_m=0.15, L0=20.0, l0=0.01, bicubic=bicubic, **kw)
+
+
+@pytest.mark.parametrize("bicubic", [True, False])
+def test_frames_ref_and_sci_fields(ctx, oracle, bicubic):
+    c = _cfg1_like(oracle, bicubic=bicubic, unwrap_mode=[0, 1])
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=3)
+
+
+def test_frames_two_directions_many_wavelengths(ctx, oracle):
+    lams = np.linspace(0.5e-6, 2.2e-6, 9)
+    c = _cfg1_like(oracle, N=64, Nout=64, M=128, lambdas=lams, unwrap_mode=[0] * 9,
+                   directions=((0.0, 0.0), (2.9e-4, -2.9e-4)))
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
+
+
+def test_frames_merged_superlayers(ctx, oracle):
+    # layers closer than FRESNEL_PROPAGATION_BIN share a super-layer
+    c = build_case(oracle, N=128, Nout=100, M=256, alts=[0.0, 500.0, 1000.0, 4000.0, 4300.0],
+                   cn2=[0.4, 0.15, 0.1, 0.2, 0.15], speeds=[5, 10, 15, 20, 25],
+                   dirs=[0.1, 1.0, 2.0, 3.0, 4.0], lambdas=[0.5e-6, 0.9e-6], lam_ref=0.5e-6, dx=0.01,
+                   dt=0.001, r0_m=0.12, L0=20.0, l0=0.01, unwrap_mode=[0, 1], bin_m=1000.0)
+    assert len(c.slayer_alt) < c.L
+    gs = gpu_sim(ctx, c)
+    assert gs.gaps() == len(c.slayer_alt) - 1
+    _compare(c, oracle_sims(oracle, c), gs, nframes=2)
+
+
+def test_phase_only_and_no_fresnel(ctx, oracle):
+    c = _cfg1_like(oracle, unwrap_mode=[0, 1])
+    _compare(c, oracle_sims(oracle, c, fresnel=False), gpu_sim(ctx, c, fresnel=False), nframes=2)
+    c2 = _cfg1_like(oracle, unwrap_mode=[0, 0])
+    os2 = oracle_sims(oracle, c2, amplitude=False)
+    gs2 = gpu_sim(ctx, c2, amplitude=False)
+    for _ in range(2):
+        pg, ag = gs2.frame()
+        _, _, po, ao = os2[0].frame(want_field=False)
+        assert rel_l2(pg.cpu().numpy()[0], po) < 1e-6
+        assert np.all(ag.cpu().numpy() == 1.0)
+
+
+def test_wrap_around_master(ctx, oracle):
+    # fast wind: positions wrap around the periodic master within a few frames
+    c = build_case(oracle, N=64, Nout=64, M=64, alts=[0.0, 3000.0], cn2=[0.6, 0.4], speeds=[400.0, 900.0],
+                   dirs=[2.5, -1.0], lambdas=[0.6e-6], lam_ref=0.6e-6, dx=0.05, dt=0.01, r0_m=0.2,
+                   L0=10.0, l0=0.05)
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=4)
