@@ -39,6 +39,7 @@ _SIGS = {
     "atsim_sync": (C.c_int, [C.c_void_p]),
     "atsim_stream": (C.c_void_p, [C.c_void_p]),
     "atsim_device_ok": (C.c_int, [C.c_void_p]),
+    "atsim_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "atsim_fft2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int]),
     "atsim_permut": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long]),
     "atsim_fresnel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long,
@@ -63,6 +64,11 @@ _SIGS = {
     "atsim_sim_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "atsim_sim_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "atsim_sim_gaps": (C.c_long, [C.c_void_p]),
+    "atsim_sim_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "atsim_sim_synth_masters": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "atsim_sim_normalize_masters": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_double,
+                                              C.POINTER(C.c_double)]),
+    "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
 }
 
 

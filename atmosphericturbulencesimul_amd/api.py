@@ -177,6 +177,11 @@ class Context:
         return s
 
 
+def hip_copy(ctx, dst, src, nbytes):
+    """device/host copy through the library's runtime (hipMemcpyDefault)"""
+    ctx._chk(ctx.L.atsim_copy(ctx.h, C.c_void_p(dst), C.c_void_p(src), nbytes))
+
+
 def superlayers(alt, cn2, bin_m):
     """atsim_superlayers: super-layer altitudes and layer -> super-layer index"""
     L_ = _lib.lib()
@@ -278,3 +283,34 @@ class Simulation:
 
     def gaps(self):
         return self.ctx.L.atsim_sim_gaps(self.h)
+
+    def synth_masters(self, seed, outerscale_px, innerscale_px):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(outerscale_px, np.float64), (self.L,)))
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(innerscale_px, np.float64), (self.L,)))
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self.ctx._chk(self.ctx.L.atsim_sim_synth_masters(self.h, seed, dp(a), dp(b)))
+
+    def normalize_masters(self, r0_goal_px, cn2, zangle=0.0):
+        cn2 = np.ascontiguousarray(cn2, np.float64)
+        r0 = np.empty(self.L)
+        dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+        self.ctx._chk(self.ctx.L.atsim_sim_normalize_masters(self.h, r0_goal_px, dp(cn2), zangle, dp(r0)))
+        return r0
+
+    def master(self, layer):
+        """torch float32 (M, M) copy of a layer's master"""
+        torch = _torch()
+        out = torch.empty((self.M, self.M), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        hip_copy(self.ctx, out.data_ptr(), self.master_ptr(layer), 4 * self.M * self.M)
+        return out
+
+    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3}
+
+    def profile(self, enable=True):
+        self.ctx._chk(self.ctx.L.atsim_sim_profile(self.h, int(enable)))
+
+    def profile_read(self, kind):
+        ms = C.c_double()
+        n = C.c_long()
+        self.ctx._chk(self.ctx.L.atsim_sim_profile_read(self.h, self.PROF_KINDS[kind], C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -110,6 +110,13 @@ int atsim_sync(atsim_ctx* c) {
 
 void* atsim_stream(atsim_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
+    if (!c || !dst || !src) return ATSIM_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dst, src, nbytes, hipMemcpyDefault));
+    return ATSIM_OK;
+}
+
 int atsim_device_ok(atsim_ctx* c) {
     if (!c) return 0;
     hipDeviceProp_t p;
@@ -380,7 +387,40 @@ struct atsim_sim {
     int* flist_dev = nullptr;
     int nblockfields = 0;
     int xs = 0;
+    // per-launch event timing (bench instrumentation)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> prof_recs;  // (kind, index of the first event of the pair)
 };
+
+enum { PROF_ROW = 0, PROF_COL = 1, PROF_EXTRACT = 2, PROF_BLOCK = 3, PROF_KINDS = 4 };
+
+static hipEvent_t prof_event(atsim_sim* s) {
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        s->ev_pool.push_back(e);
+    }
+    return s->ev_pool[s->ev_used++];
+}
+
+// record an event before a launch of `kind`; returns the pair index or -1
+static long prof_begin(atsim_sim* s, int kind) {
+    if (!s->prof) return -1;
+    const size_t i0 = s->ev_used;
+    hipEvent_t a = prof_event(s);
+    hipEvent_t b = prof_event(s);
+    if (!a || !b) return -1;
+    (void)hipEventRecord(a, s->ctx->stream);
+    s->prof_recs.push_back({kind, i0});
+    return (long)i0;
+}
+
+static void prof_end(atsim_sim* s, long i0) {
+    if (i0 < 0) return;
+    (void)hipEventRecord(s->ev_pool[(size_t)i0 + 1], s->ctx->stream);
+}
 
 static void sim_free(atsim_sim* s) {
     if (!s) return;
@@ -401,6 +441,7 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->bin);
     hipFree(s->ch);
     hipFree(s->flist_dev);
+    for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -600,8 +641,12 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
             sa.lfirst = s->groups[g].lfirst;
             sa.lcount = s->groups[g].lcount;
             const float2* tab = s->tf + (size_t)g * W * s->N;
+            long pr = prof_begin(s, PROF_ROW);
             HIPCHK(c, launch_row(s->N, OP_FRESNEL, 0, g == 0, s->fields, F, tab, s->tw, sa, c->stream));
+            prof_end(s, pr);
+            long pc = prof_begin(s, PROF_COL);
             HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream));
+            prof_end(s, pc);
         }
     }
     ExtractArgs a{};
@@ -630,12 +675,85 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     a.xs = s->xs;
     if (s->nblockfields > 0 && s->xs > 0)
         HIPCHK(c, hipMemsetAsync(s->bsum, 0, sizeof(double) * F * s->xs * s->xs * 2, c->stream));
+    long pe = prof_begin(s, PROF_EXTRACT);
     HIPCHK(c, launch_extract(a, d_field != nullptr, c->stream));
+    prof_end(s, pe);
     if (s->nblockfields > 0) {
+        long pb = prof_begin(s, PROF_BLOCK);
         HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->flist_dev, s->nblockfields, s->xs, c->stream));
         HIPCHK(c, launch_block_apply(d_pha, s->dsave, s->bin, s->flist_dev, s->nblockfields, (int)s->Nout, s->xs,
                                      c->stream));
+        prof_end(s, pb);
     }
+    return ATSIM_OK;
+}
+
+int atsim_sim_synth_masters(atsim_sim* s, uint64_t seed, const double* outerscale_px, const double* innerscale_px) {
+    if (!s || !outerscale_px || !innerscale_px) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    if (!supported_n(s->M)) return fail(c, ATSIM_EUNSUPPORTED, "synth_masters: M=%ld unsupported", s->M);
+    float2* noise = nullptr;
+    HIPCHK(c, hipMalloc(&noise, sizeof(float2) * s->M * s->M));
+    int r = ATSIM_OK;
+    for (long l = 0; l < s->L && r == ATSIM_OK; l++) {
+        r = atsim_noise_c64(c, noise, s->M * s->M, seed, (uint32_t)l);
+        if (r == ATSIM_OK)
+            r = atsim_make_master(c, noise, s->M, outerscale_px[l], innerscale_px[l], 0.0,
+                                  s->masters + (size_t)l * s->M * s->M, nullptr, nullptr, nullptr);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(noise);
+    return r;
+}
+
+int atsim_sim_normalize_masters(atsim_sim* s, double r0_goal_px, const double* cn2, double zangle,
+                                double* r0_measured) {
+    if (!s || !cn2 || !(r0_goal_px > 0)) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long dpix = 50 < s->M / 2 ? 50 : s->M / 4;
+    double r0tot = 0.0;
+    for (long l = 0; l < s->L; l++) {
+        double r0 = 0.0;
+        if (int r = atsim_r0_measure(c, s->masters + (size_t)l * s->M * s->M, s->M, dpix, &r0)) return r;
+        if (r0_measured) r0_measured[l] = r0;
+        r0tot += r0;
+    }
+    // AT.c:2271-2285: one factor for all masters so the mean r0 hits the goal
+    const double tmp1 = pow((r0tot / s->L) / r0_goal_px, 5.0 / 6.0);
+    for (long l = 0; l < s->L; l++) {
+        float* m = s->masters + (size_t)l * s->M * s->M;
+        if (int r = atsim_scale(c, m, s->M * s->M, tmp1)) return r;
+        // AT.c:2339-2351: layer weight sqrt(cn2 / cos z)
+        if (int r = atsim_scale(c, m, s->M * s->M, sqrt(cn2[l] / cos(zangle)))) return r;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_sim_profile(atsim_sim* s, int enable) {
+    if (!s) return ATSIM_EINVAL;
+    HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    s->prof = enable != 0;
+    s->ev_used = 0;
+    s->prof_recs.clear();
+    return ATSIM_OK;
+}
+
+int atsim_sim_profile_read(atsim_sim* s, int kind, double* ms, long* count) {
+    if (!s || kind < 0 || kind >= PROF_KINDS) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    long n = 0;
+    for (auto& r : s->prof_recs) {
+        if (r.first != kind) continue;
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, s->ev_pool[r.second], s->ev_pool[r.second + 1]));
+        tot += t;
+        n++;
+    }
+    if (ms) *ms = tot;
+    if (count) *count = n;
     return ATSIM_OK;
 }
 

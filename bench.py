@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Wavefront-frame throughput of the MI355X path on one GPU.
+
+Workload (headline): 2048^2 propagation grid and output, 1 cm pixels,
+8192^2 von Karman masters (L0 = 20 m, l0 = 1 cm) synthesised on the GPU from
+seeded Philox noise, 10 layers at h_k = 20k/9 km with Cn2 ~ exp(-h / 2 km),
+r0 = 0.12 m at 0.5 um, winds 5..40 m/s, 8 wavelengths 0.5..2.2 um, one
+direction, Catmull-Rom sampling, Fresnel propagation between the 10
+super-layers (9 gaps).  A step is one time step: every layer advanced, all 8
+wavelength fields propagated and extracted (OPD + amplitude) = 8
+wavefront-frames.
+
+Printed JSON line: value = wavefront-frames/s with everything resident in
+HBM; roofline of the dominant kernel from HIP events recorded around each of
+its launches inside the timed region; cpu_baseline = the CPU oracle on a
+bounded sample of the same workload (single core), which also gives the
+FP64 error of one full-size field.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "wavefront-frames/sec (λ×dir) at 2048², 10 layers; FFT HBM GB/s; err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def headline_workload():
+    L = 10
+    alts = np.array([20000.0 * k / 9 for k in range(L)])
+    cn2 = np.exp(-alts / 2000.0)
+    cn2 /= cn2.sum()
+    return dict(
+        N=2048, Nout=2048, M=8192, L=L, W=8, D=1, dx=0.01, dt=0.001, r0=0.12, lam_ref=0.5e-6, L0=20.0, l0=0.01,
+        alts=alts, cn2=cn2, speeds=np.linspace(5.0, 40.0, L),
+        dirs=np.array([(2.399963229728653 * k) % (2 * math.pi) for k in range(L)]),
+        lambdas=np.linspace(0.5e-6, 2.2e-6, 8), seed=0xA75 + 4000, bin_m=1000.0)
+
+
+def make_sim(ctx, wl, M=None, lambdas=None, unwrap=None):
+    from atmosphericturbulencesimul_amd.api import Simulation, superlayers
+    M = M or wl["M"]
+    lambdas = wl["lambdas"] if lambdas is None else np.asarray(lambdas)
+    W = lambdas.size
+    L = wl["L"]
+    sa, sli = superlayers(wl["alts"], wl["cn2"], wl["bin_m"])
+    x0 = np.full((1, L), 0.05 * M)
+    y0 = np.full((1, L), 0.05 * M)
+    vx = wl["speeds"] * np.cos(wl["dirs"]) / wl["dx"]
+    vy = wl["speeds"] * np.sin(wl["dirs"]) / wl["dx"]
+    # phase scale lambda_ref / lambda; the refractivity ratio N(h, lambda)/N(h, lambda_ref)
+    # of the reference's atmosphere model is taken as 1 here
+    scoeff = np.array([[wl["lam_ref"] / lam] * L for lam in lambdas])
+    if unwrap is None:
+        unwrap = [0 if abs(lam - wl["lam_ref"]) < 1e-12 else 1 for lam in lambdas]
+    sim = Simulation(ctx, wl["N"], wl["Nout"], M, sa, sli, lambdas, scoeff, unwrap, x0, y0, vx, vy,
+                     wl["dx"], wl["dt"], 0.0, True)
+    return sim, dict(sa=sa, sli=sli, x0=x0, y0=y0, vx=vx, vy=vy, scoeff=scoeff, unwrap=np.asarray(unwrap))
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprof PMC summary"""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(p):
+        return None
+    out = {}
+    for line in open(p):
+        if line.startswith("|") and "bytes_per_launch" not in line and "---" not in line:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 2:
+                try:
+                    out[cells[0]] = float(cells[-1])
+                except ValueError:
+                    pass
+    return out or None
+
+
+def cpu_baseline_and_error(ctx, wl, masters_gpu, frames=2):
+    """Oracle (FP64, one core) on a bounded sample: the same N = 2048, 10-layer,
+    bicubic, Fresnel layer loop for one wavelength (0.5 um) with 2048^2 masters
+    cut from the GPU's; the GPU runs the same sample for the FP64 error."""
+    import torch
+    from oracle import pyoracle as po
+    M = wl["N"]
+    L = wl["L"]
+    ms = np.stack([m[:M, :M].cpu().numpy() for m in masters_gpu]).astype(np.float32)
+    sim, p = make_sim(ctx, wl, M=M, lambdas=[wl["lam_ref"]], unwrap=[0])
+    for l in range(L):
+        sim.set_master(l, ms[l])
+    osim = po.Sim(wl["N"], wl["Nout"], ms.astype(np.float64), p["sa"], p["sli"], [wl["lam_ref"]], p["scoeff"],
+                  [0], p["x0"][0], p["y0"][0], p["vx"], p["vy"], wl["dx"], wl["dt"], 0.0, True)
+    field = torch.empty((1, 1, wl["N"], wl["N"]), dtype=torch.complex64, device="cuda")
+    errs = dict(field=0.0, amp=0.0, opd=0.0, unwrap_mismatch=0)
+    t_cpu = 0.0
+    for _ in range(frames):
+        pha_g, amp_g = sim.frame(field)
+        t0 = time.perf_counter()
+        E_o, _, pha_o, amp_o = osim.frame()
+        t_cpu += time.perf_counter() - t0
+        rl = lambda a, b: float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+        errs["field"] = max(errs["field"], rl(field.cpu().numpy()[0, 0], E_o[0]))
+        errs["amp"] = max(errs["amp"], rl(amp_g.cpu().numpy()[0, 0], amp_o[0]))
+        pg = pha_g.cpu().numpy()[0, 0]
+        bad = np.abs(pg - pha_o[0]) > 1.0
+        errs["unwrap_mismatch"] = max(errs["unwrap_mismatch"], int(bad.sum()))
+        errs["opd"] = max(errs["opd"], rl(pg[~bad], pha_o[0][~bad]))
+    sim.close()
+    cpu = {
+        "value": frames / t_cpu,
+        "unit": "wavefront-frames/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{frames} frames x 1 wavelength of the 2048^2, 10-layer, 9-gap bicubic layer loop "
+                  f"(FP64 oracle, own radix-2 FFT; masters 2048^2 cut from the GPU's), {t_cpu:.1f} s on "
+                  f"{platform.processor() or platform.machine()}",
+    }
+    errs["case"] = "one 0.5 um field, 2048^2, 10 layers, vs FP64 oracle (max over frames)"
+    return cpu, errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / FP64 error sample")
+    ap.add_argument("--cpu-frames", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the wavefront path is built and measured on a single MI355X "
+              "(independent (lambda, direction) shards would split across GPUs with no exchange); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from atmosphericturbulencesimul_amd.api import Context
+
+    wl = headline_workload()
+    ctx = Context(0)
+    sim, _ = make_sim(ctx, wl)
+    t0 = time.perf_counter()
+    sim.synth_masters(wl["seed"], wl["L0"] / wl["dx"], wl["l0"] / wl["dx"])
+    r0_meas = sim.normalize_masters(wl["r0"] / wl["dx"], wl["cn2"])
+    t_setup = time.perf_counter() - t0
+    G = sim.gaps()
+    F = wl["W"] * wl["D"]
+
+    for _ in range(args.warmup):
+        sim.frame(sync=False)
+    ctx.sync()
+    sim.profile(True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.frame(sync=False)
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = {k: sim.profile_read(k) for k in ("row", "col", "extract", "block")}
+    sim.profile(False)
+
+    N = wl["N"]
+    pass_bytes = 16.0 * N * N * F  # one read + one write of every field per pass
+    avg = {k: (ms / n if n else float("nan")) for k, (ms, n) in prof.items()}
+    dom = "row" if prof["row"][0] >= prof["col"][0] else "col"
+    achieved = pass_bytes / (avg[dom] * 1e-3) / 1e9
+    fft_ms = prof["row"][0] + prof["col"][0]
+    fft_gbs = pass_bytes * (prof["row"][1] + prof["col"][1]) / (fft_ms * 1e-3) / 1e9
+    value = args.steps * F / elapsed
+    kernel_names = {"row": "k_row<2048, OP_FRESNEL> (x pass: screen phase, FFT, transfer function, IFFT)",
+                    "col": "k_col<2048, OP_FRESNEL> (y pass: FFT, transfer function, IFFT)"}
+    traffic = None
+    tr = read_traffic()
+    if tr and dom in tr:
+        traffic = tr[dom]
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "wavefront-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (Philox-seeded noise, masters synthesised on the GPU)",
+        "config": {
+            "workload": "2048^2 grid, 10 layers (9 Fresnel gaps), 8 wavelengths 0.5-2.2 um, 1 direction, "
+                        "8192^2 masters, bicubic, block unwrap on 7 science wavelengths",
+            "N": N, "Nout": wl["Nout"], "M": wl["M"], "layers": wl["L"], "gaps": int(G),
+            "wavelengths": wl["W"], "directions": wl["D"], "wavefront_frames_per_step": F,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel_names[dom],
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": pass_bytes,
+            "avg_launch_ms": round(avg[dom], 5),
+        },
+        "fft_hbm_gbs": round(fft_gbs, 1),
+        "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "setup_s": round(t_setup, 3),
+        "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
+    }
+    if not args.no_cpu:
+        masters = [sim.master(l) for l in range(wl["L"])]
+        sim.close()
+        cpu, errs = cpu_baseline_and_error(ctx, wl, masters, frames=args.cpu_frames)
+        line["cpu_baseline"] = cpu
+        line["err_vs_fp64"] = errs
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -17,6 +17,7 @@
 #ifndef ATSIM_H
 #define ATSIM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -41,6 +42,8 @@ const char *atsim_error(const atsim_ctx *ctx);
 int atsim_sync(atsim_ctx *ctx);
 /* HIP stream (hipStream_t) all work of this context is queued on */
 void *atsim_stream(atsim_ctx *ctx);
+/* synchronous copy between host and/or device buffers (unified addressing) */
+int atsim_copy(atsim_ctx *ctx, void *dst, const void *src, size_t nbytes);
 /* 1 if the library was built with code for the current device */
 int atsim_device_ok(atsim_ctx *ctx);
 
@@ -138,6 +141,17 @@ void atsim_sim_destroy(atsim_sim *sim);
 float *atsim_sim_master(atsim_sim *sim, long layer);
 /* copy an M x M float master (host or device pointer) into layer `layer` */
 int atsim_sim_set_master(atsim_sim *sim, long layer, const float *src);
+/* Fill every layer's master with make_master_turbulence_screen from the
+ * Philox noise stream `layer` of `seed` (outer/inner scales in pixels per
+ * layer, AT.c:2204-2226). */
+int atsim_sim_synth_masters(atsim_sim *sim, uint64_t seed, const double *outerscale_px,
+                            const double *innerscale_px);
+/* Per-run normalisation (AT.c:2246-2363): r0 of every master from its
+ * 50-pixel x structure function, all masters scaled by (mean r0 /
+ * r0_goal)^(5/6), then layer l by sqrt(cn2[l] / cos zangle).  r0_measured
+ * (nullable, L) receives the r0 values before scaling [pixels]. */
+int atsim_sim_normalize_masters(atsim_sim *sim, double r0_goal_px, const double *cn2, double zangle,
+                                double *r0_measured);
 /* Advance every layer position by one time step (AT.c:2700-2755) and compute
  * one frame for all D x W fields: d_pha / d_amp are device float arrays
  * [D][W][Nout][Nout] (phase in rad, unwrapped; amplitude); d_field (nullable)
@@ -145,8 +159,14 @@ int atsim_sim_set_master(atsim_sim *sim, long layer, const float *src);
 int atsim_sim_frame(atsim_sim *sim, float *d_pha, float *d_amp, void *d_field);
 /* current FP64 positions [D][L] (host arrays, nullable) */
 int atsim_sim_positions(atsim_sim *sim, double *xpos, double *ypos);
-/* number of Fresnel gaps per frame and of row/column passes per frame */
+/* number of Fresnel gaps per frame (row + column pass each) */
 long atsim_sim_gaps(atsim_sim *sim);
+/* Per-launch HIP-event timing of the frame kernels, for benchmarking.
+ * enable = 1 starts (and clears) the record, 0 stops it.  kind: 0 row pass,
+ * 1 column pass, 2 extraction, 3 block unwrap; returns the summed duration
+ * [ms] and the number of launches recorded since the last enable. */
+int atsim_sim_profile(atsim_sim *sim, int enable);
+int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
 
 #ifdef __cplusplus
 }
