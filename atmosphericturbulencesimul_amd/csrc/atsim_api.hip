@@ -642,7 +642,7 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
             sa.lcount = s->groups[g].lcount;
             const float2* tab = s->tf + (size_t)g * W * s->N;
             long pr = prof_begin(s, PROF_ROW);
-            HIPCHK(c, launch_row(s->N, OP_FRESNEL, 0, g == 0, s->fields, F, tab, s->tw, sa, c->stream));
+            HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, c->stream));
             prof_end(s, pr);
             long pc = prof_begin(s, PROF_COL);
             HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream));

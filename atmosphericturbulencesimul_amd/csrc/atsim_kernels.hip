@@ -35,6 +35,12 @@ __device__ __forceinline__ int wrap_m(int x, int M) {
     return x;
 }
 
+// x mod M for any int x (staging loops)
+__device__ __forceinline__ int mod_m(int x, int M) {
+    int r = x % M;
+    return r < 0 ? r + M : r;
+}
+
 // separable 4 x 4 interpolation of layer l's master around (x0 + i, y0 + j), FP64
 __device__ __forceinline__ double sample_layer(const float* __restrict__ m, int M, const LayerSample& p,
                                                int i, int j) {
@@ -163,6 +169,114 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
     for (int q = 0; q < 16; q++) base[(size_t)reg_index<N, RO>(t, q) * N] = v[q];
 }
 
+// Fresnel x pass for up to MW wavelength fields of the same direction and
+// spatial row j: the screen phase of the layer group is sampled once per
+// pixel (FP64, master rows staged in LDS) and shared by the wavelengths,
+// then each field is multiplied by e^{i s_w v}, transformed along x,
+// multiplied by its transfer factor and transformed back.
+template <int N>
+struct LamCfg {
+    static constexpr int T = N / 16;
+    static constexpr int MW0 = 16384 / N;
+    static constexpr int MW = MW0 < 8 ? (MW0 < 1 ? 1 : MW0) : 8;
+    static constexpr int SEG = N < 2048 ? N : 2048;
+};
+
+template <int N, bool GEN>
+__global__ void __launch_bounds__(LamCfg<N>::MW * (N / 16))
+k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const float2* __restrict__ tw,
+              ScreenArgs sa) {
+    using P = Plan<N>;
+    constexpr int T = P::T, MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG, SW = SEG + 4;
+    constexpr int RI = P::first(-1), RO = P::last(+1), RL = P::last(-1);
+    extern __shared__ float2 lds[];  // max(mwc * P::LDS float2, screen staging)
+    float* S = reinterpret_cast<float*>(lds);            // [4][SW] staged master rows
+    double* V = reinterpret_cast<double*>(S + 4 * SW);  // [SEG] sampled screen
+    const int mwc = blockDim.x / T;                      // wavelengths per workgroup (<= MW)
+    const int j = blockIdx.x, d = blockIdx.y, w0 = blockIdx.z * mwc;
+    const int nw = min(mwc, sa.W - w0);
+    const int lw = threadIdx.x / T, t = threadIdx.x - lw * T;
+    const bool active = lw < nw;
+    const int w = min(w0 + lw, sa.W - 1);
+    const int f = d * sa.W + w;
+    const int nthr = blockDim.x;
+    const int M = (int)sa.M;
+    double ph[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) ph[q] = 0.0;
+    for (int k = 0; k < sa.lcount; k++) {
+        const int l = sa.lfirst + k;
+        const LayerSample p = sa.ls[d * sa.L + l];
+        const float* m = sa.masters + (size_t)l * M * M;
+        const double sw = sa.s[w * sa.L + l];
+        for (int seg0 = 0; seg0 < N; seg0 += SEG) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < 4 * (SEG + 3); e += nthr) {
+                const int b = e / (SEG + 3), c = e - b * (SEG + 3);
+                S[b * SW + c] = m[(size_t)mod_m(p.y0 + j - 1 + b, M) * M + mod_m(p.x0 + seg0 - 1 + c, M)];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < SEG; i += nthr) {
+                double v = 0.0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const float* r = S + b * SW + i;
+                    double h = p.wx[0] * (double)r[0];
+                    h = fma(p.wx[1], (double)r[1], h);
+                    h = fma(p.wx[2], (double)r[2], h);
+                    h = fma(p.wx[3], (double)r[3], h);
+                    v = fma(p.wy[b], h, v);
+                }
+                V[i] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int idx = reg_index<N, RI>(t, q) - seg0;
+                if (idx >= 0 && idx < SEG) ph[q] = fma(sw, V[idx], ph[q]);
+            }
+        }
+    }
+    float2* rowp = data + ((size_t)f * N + j) * N;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const float2 e = expi_d(ph[q]);
+        v[q] = GEN ? e : (active ? cmul(rowp[reg_index<N, RI>(t, q)], e) : e);
+    }
+    float2* Lr = lds + lw * P::LDS;
+    fft_regs<N, -1, 1>(v, Lr, t, 0, tw);
+    const float2* h = tf + (size_t)w * N;
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t, q)]);
+    fft_regs<N, +1, 1>(v, Lr, t, 0, tw);
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) rowp[reg_index<N, RO>(t, q)] = v[q];
+    }
+}
+
+template <int N>
+static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const float2* tf, const float2* tw,
+                                       const ScreenArgs& sa, hipStream_t st) {
+    constexpr int MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG;
+    const int mwc = sa.W < MW ? sa.W : MW;
+    size_t lds = (size_t)mwc * Plan<N>::LDS * sizeof(float2);
+    const size_t staging = (size_t)16 * (SEG + 4) + (size_t)8 * SEG;
+    if (lds < staging) lds = staging;
+    dim3 grid(N, D, (sa.W + mwc - 1) / mwc), block(mwc * (N / 16));
+    static bool attr = false;
+    if (!attr) {
+        const int mx = (int)(160 * 1024);
+        (void)hipFuncSetAttribute((const void*)k_row_fresnel<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        (void)hipFuncSetAttribute((const void*)k_row_fresnel<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        attr = true;
+    }
+    if (gen) k_row_fresnel<N, true><<<grid, block, lds, st>>>(data, tf, tw, sa);
+    else k_row_fresnel<N, false><<<grid, block, lds, st>>>(data, tf, tw, sa);
+    return hipGetLastError();
+}
+
 template <int N>
 static hipError_t row_dispatch(int op, int dir, bool gen, float2* data, long F, const float2* tf, int W,
                                const float2* tw, const ScreenArgs& sa, hipStream_t st) {
@@ -198,6 +312,17 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
     switch (n) {
 #define X(NN) \
     case NN: return row_dispatch<NN>(op, dir, gen, data, F, tf, W, tw, sa, st);
+        ATSIM_FOR_EACH_N(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
+                              const ScreenArgs& sa, hipStream_t st) {
+    switch (n) {
+#define X(NN) \
+    case NN: return row_fresnel_dispatch<NN>(gen, data, D, tf, tw, sa, st);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;
@@ -466,49 +591,80 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 }
 
 // ------------------------------------------------------------------ extraction
-// One workgroup per (row, direction, wavelength chunk of <= 8).  Every layer
-// is sampled once per pixel (FP64) and shared by the chunk's wavelengths.
+// One workgroup per (256-pixel row segment, row, direction x wavelength
+// chunk of <= 8), one pixel per thread.  For every layer the 4 master rows
+// under the segment are staged in LDS and the pixel's sample (FP64) is shared
+// by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l and the
+// screen of the last layer group, applied here after the last Fresnel gap.
+constexpr int EXT_T = 256;
 template <bool FULL>
-__global__ void __launch_bounds__(256) k_extract(ExtractArgs a) {
-    extern __shared__ double bs[];  // [8][2][xs]
-    const int row = blockIdx.x, d = blockIdx.y;
-    a.w_begin = blockIdx.z * 8;
+__global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
+    __shared__ float S[4][EXT_T + 4];
+    __shared__ double bs[8][2][16];
+    const int nchunks = (a.W + 7) / 8;
+    const int seg = blockIdx.x, row = blockIdx.y;
+    const int d = blockIdx.z / nchunks, chunk = blockIdx.z - d * nchunks;
+    const int w_begin = chunk * 8;
+    const int Wc = min(8, a.W - w_begin);
     const int o = (a.N - a.Nout) / 2;
     const int jf = FULL ? row : row + o;
     const int jj = jf - o;
     const bool rowin = jj >= 0 && jj < a.Nout;
     const int ncols = FULL ? a.N : a.Nout;
-    const int Wc = min(8, a.W - a.w_begin);
+    const int c0 = seg * EXT_T;
+    const int cc = c0 + threadIdx.x;
+    const bool valid = cc < ncols;
+    const int xfirst = FULL ? c0 : c0 + o;  // full-grid x of the segment's first pixel
+    const int ifl = xfirst + threadIdx.x;
+    const int ii = ifl - o;
+    const bool colin = valid && ii >= 0 && ii < a.Nout;
     const int xs = a.xs;
     const int jj2 = rowin ? (int)((double)jj / a.Nout * xs) : -1;
-    const bool blockrow = rowin && jj2 >= 0 && jj2 < xs;
+    const bool blockrow = rowin && jj2 >= 0 && jj2 < xs && a.amplitude && a.bsum != nullptr;
+    const int ii_lo = max(xfirst - o, 0);
+    const int b0 = (xs > 0 && ii_lo < a.Nout) ? (int)((double)ii_lo / a.Nout * xs) : 0;
     const int L = a.all.L;
-    for (int k = threadIdx.x; k < 16 * xs; k += blockDim.x) bs[k] = 0.0;
-    __syncthreads();
-    for (int cc = threadIdx.x; cc < ncols; cc += blockDim.x) {
-        const int ifl = FULL ? cc : cc + o;
-        const int ii = ifl - o;
-        const bool colin = ii >= 0 && ii < a.Nout;
-        double PHI[8], PL[8];
+    const int M = (int)a.all.M;
+    for (int k = threadIdx.x; k < 8 * 2 * 16; k += EXT_T) (&bs[0][0][0])[k] = 0.0;
+    double PHI[8], PL[8];
 #pragma unroll
-        for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
-        for (int l = 0; l < L; l++) {
-            const LayerSample p = a.all.ls[d * L + l];
-            const double v = sample_layer(a.all.masters + (size_t)l * a.all.M * a.all.M, (int)a.all.M, p, ifl, jf);
+    for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
+    for (int l = 0; l < L; l++) {
+        const LayerSample p = a.all.ls[d * L + l];
+        const float* m = a.all.masters + (size_t)l * M * M;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 4 * (EXT_T + 3); e += EXT_T) {
+            const int b = e / (EXT_T + 3), c = e - b * (EXT_T + 3);
+            S[b][c] = m[(size_t)mod_m(p.y0 + jf - 1 + b, M) * M + mod_m(p.x0 + xfirst - 1 + c, M)];
+        }
+        __syncthreads();
+        if (valid) {
+            double v = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const float* r = &S[b][threadIdx.x];
+                double h = p.wx[0] * (double)r[0];
+                h = fma(p.wx[1], (double)r[1], h);
+                h = fma(p.wx[2], (double)r[2], h);
+                h = fma(p.wx[3], (double)r[3], h);
+                v = fma(p.wy[b], h, v);
+            }
             const bool inlast = l >= a.last_first && l < a.last_first + a.last_count;
 #pragma unroll
             for (int w = 0; w < 8; w++) {
                 if (w < Wc) {
-                    const double sv = a.all.s[(a.w_begin + w) * L + l] * v;
+                    const double sv = a.all.s[(w_begin + w) * L + l] * v;
                     PHI[w] += sv;
                     if (inlast) PL[w] += sv;
                 }
             }
         }
+    }
+    if (valid) {
 #pragma unroll
         for (int w = 0; w < 8; w++) {
             if (w >= Wc) continue;
-            const int f = d * a.W + a.w_begin + w;
+            const int f = d * a.W + w_begin + w;
             float2 E = a.fields ? a.fields[((size_t)f * a.N + jf) * a.N + ifl] : make_float2(1.f, 0.f);
             if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
             if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
@@ -522,38 +678,39 @@ __global__ void __launch_bounds__(256) k_extract(ExtractArgs a) {
             const double re = E.x, im = E.y;
             a.amp[oi] = (float)sqrt(re * re + im * im);
             const double ph = (double)atan2f(E.y, E.x);
-            if (a.unwrap_mode[a.w_begin + w] == 0) {
+            if (a.unwrap_mode[w_begin + w] == 0) {
                 a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] / 2.0 / M_PI));
             } else {
                 a.pha[oi] = (float)ph;
                 if (a.dsave) a.dsave[oi] = PHI[w] - ph;
                 const int ii2 = (int)((double)ii / a.Nout * xs);
-                if (blockrow && ii2 < xs) {
+                if (blockrow && ii2 < xs && ii2 - b0 < 16) {
                     const float2 cs = expi_d(PHI[w] - ph);
-                    atomicAdd(&bs[(w * 2 + 0) * xs + ii2], (double)cs.x);
-                    atomicAdd(&bs[(w * 2 + 1) * xs + ii2], (double)cs.y);
+                    atomicAdd(&bs[w][0][ii2 - b0], (double)cs.x);
+                    atomicAdd(&bs[w][1][ii2 - b0], (double)cs.y);
                 }
             }
         }
     }
     __syncthreads();
-    if (!blockrow || !a.amplitude || a.bsum == nullptr) return;
+    if (!blockrow) return;
     const int nb = xs * xs;
-    for (int k = threadIdx.x; k < Wc * xs; k += blockDim.x) {
-        const int w = k / xs, ii2 = k - w * xs;
-        if (a.unwrap_mode[a.w_begin + w] == 0) continue;
-        const int f = d * a.W + a.w_begin + w;
-        double* dst = a.bsum + ((size_t)f * nb + (size_t)jj2 * xs + ii2) * 2;
-        atomicAdd(dst, bs[(w * 2 + 0) * xs + ii2]);
-        atomicAdd(dst + 1, bs[(w * 2 + 1) * xs + ii2]);
+    for (int k = threadIdx.x; k < Wc * 16; k += EXT_T) {
+        const int w = k >> 4, lb = k & 15;
+        if (a.unwrap_mode[w_begin + w] == 0 || b0 + lb >= xs) continue;
+        if (bs[w][0][lb] == 0.0 && bs[w][1][lb] == 0.0) continue;
+        const int f = d * a.W + w_begin + w;
+        double* dst = a.bsum + ((size_t)f * nb + (size_t)jj2 * xs + b0 + lb) * 2;
+        atomicAdd(dst, bs[w][0][lb]);
+        atomicAdd(dst + 1, bs[w][1][lb]);
     }
 }
 
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st) {
-    dim3 grid(full ? a.N : a.Nout, a.D, (a.W + 7) / 8), block(256);
-    size_t sh = (size_t)16 * (a.xs > 0 ? a.xs : 1) * sizeof(double);
-    if (full) k_extract<true><<<grid, block, sh, st>>>(a);
-    else k_extract<false><<<grid, block, sh, st>>>(a);
+    const int ncols = full ? a.N : a.Nout;
+    dim3 grid((ncols + EXT_T - 1) / EXT_T, full ? a.N : a.Nout, a.D * ((a.W + 7) / 8)), block(EXT_T);
+    if (full) k_extract<true><<<grid, block, 0, st>>>(a);
+    else k_extract<false><<<grid, block, 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -566,14 +723,17 @@ __constant__ int c_pairs[12][6] = {
     {2, 2, 0, 0, 2, 2}, {0, 2, 2, 0, 2, 2}, {0, 1, 2, 0, 2, 1}, {2, 1, 0, 0, 2, 1}};
 
 __global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ bsum, double* __restrict__ bin,
-                                                    double* __restrict__ ch, const int* __restrict__ flist, int xs) {
+                                                    double* __restrict__ ch, const int* __restrict__ flist, int xs,
+                                                    int use_lds) {
     __shared__ double redd[16];
     __shared__ int redi[16];
+    extern __shared__ double dyn[];  // [nb] B (double), [nb] ch (float) when use_lds
     const int f = flist[blockIdx.x];
     const int nb = xs * xs;
     const double* S = bsum + (size_t)f * nb * 2;
-    double* B = bin + (size_t)f * nb;
-    double* Cc = ch + (size_t)f * nb;
+    double* Bg = bin + (size_t)f * nb;
+    double* B = use_lds ? dyn : Bg;
+    float* Cc = use_lds ? reinterpret_cast<float*>(dyn + nb) : reinterpret_cast<float*>(ch) + (size_t)f * nb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) B[b] = atan2(S[2 * b + 1], S[2 * b]);
     __syncthreads();
@@ -603,8 +763,8 @@ __global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ b
                     if (pv1 > pb + M_PI) c += cq;
                 }
             }
-            Cc[b] = c;
-            mx = fmax(mx, fabs(c));
+            Cc[b] = (float)c;
+            mx = fmax(mx, fabs((double)(float)c));
         }
         for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
         if (lane == 0) redd[wv] = mx;
@@ -617,7 +777,7 @@ __global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ b
         for (int b = threadIdx.x; b < nb; b += blockDim.x) {
             const int i2 = b % xs, j2 = b / xs;
             const bool interior = i2 > 1 && j2 > 1 && i2 < xs - 2 && j2 < xs - 2;
-            const double c = Cc[b];
+            const double c = (double)Cc[b];
             if (c > plim) { if (interior) cnt++; B[b] += 2.0 * M_PI; }
             if (c < -plim) { if (interior) cnt++; B[b] -= 2.0 * M_PI; }
         }
@@ -631,12 +791,21 @@ __global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ b
         else chcnt0cnt = 0;
         __syncthreads();
     }
+    if (use_lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) Bg[b] = B[b];
 }
 
 hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* field_list, int nfields, int xs,
                             hipStream_t st) {
     if (nfields <= 0 || xs <= 0) return hipSuccess;
-    k_block_fix<<<nfields, 1024, 0, st>>>(bsum, bin, ch, field_list, xs);
+    const size_t need = (size_t)xs * xs * (sizeof(double) + sizeof(float));
+    const int use_lds = need <= 150 * 1024;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_block_fix, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        attr = true;
+    }
+    k_block_fix<<<nfields, 1024, use_lds ? need : 0, st>>>(bsum, bin, ch, field_list, xs, use_lds);
     return hipGetLastError();
 }
 

@@ -17,6 +17,11 @@ bool supported_n(long n);
 // (nothing is read).
 hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, const float2* tf,
                       const float2* tw, const ScreenArgs& sa, hipStream_t st);
+// Fresnel x pass of every (direction, wavelength) field with the screen
+// phase of the layer group sa.lfirst .. + sa.lcount (sampled once per pixel
+// and direction); gen: the incoming field is 1.  Tables tf + w n.
+hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
+                              const ScreenArgs& sa, hipStream_t st);
 // Column pass: same transforms along y; no screen.
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
                       const float2* tw, hipStream_t st);

@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / FP64 error sample")
     ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the wavefront path is built and measured on a single MI355X "
@@ -146,6 +148,13 @@ def main():
 
     wl = headline_workload()
     ctx = Context(0)
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     sim, _ = make_sim(ctx, wl)
     t0 = time.perf_counter()
     sim.synth_masters(wl["seed"], wl["L0"] / wl["dx"], wl["l0"] / wl["dx"])
