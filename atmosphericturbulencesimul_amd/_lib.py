@@ -40,6 +40,8 @@ _SIGS = {
     "atsim_stream": (C.c_void_p, [C.c_void_p]),
     "atsim_device_ok": (C.c_int, [C.c_void_p]),
     "atsim_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "atsim_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "atsim_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "atsim_fft2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int]),
     "atsim_permut": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long]),
     "atsim_fresnel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long,

@@ -75,6 +75,12 @@ class Context:
         torch.cuda.current_stream(t.device).synchronize()
         return t
 
+    def set_tuning(self, key, value):
+        self._chk(self.L.atsim_set_tuning(self.h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        return self.L.atsim_get_tuning(self.h, key.encode())
+
     def device_ok(self):
         return bool(self.L.atsim_device_ok(self.h))
 

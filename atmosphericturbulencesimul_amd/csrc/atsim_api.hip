@@ -23,6 +23,9 @@ struct atsim_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::map<long, float2*> twiddles;
+    // launch-shape tuning (0 = built-in default)
+    int row_mw = 0;  // wavelengths per row-pass workgroup
+    int col_c = 0;   // columns per column-pass workgroup
 };
 
 static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
@@ -115,6 +118,28 @@ int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(dst, src, nbytes, hipMemcpyDefault));
     return ATSIM_OK;
+}
+
+int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
+    if (!c || !key) return ATSIM_EINVAL;
+    if (!strcmp(key, "row_mw")) {
+        if (value < 0 || value > 8) return fail(c, ATSIM_EINVAL, "row_mw must be in [0, 8]");
+        c->row_mw = value;
+    } else if (!strcmp(key, "col_c")) {
+        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
+            return fail(c, ATSIM_EINVAL, "col_c must be 0, 2, 4, 8 or 16");
+        c->col_c = value;
+    } else {
+        return fail(c, ATSIM_EINVAL, "unknown tuning key '%s'", key);
+    }
+    return ATSIM_OK;
+}
+
+int atsim_get_tuning(atsim_ctx* c, const char* key) {
+    if (!c || !key) return ATSIM_EINVAL;
+    if (!strcmp(key, "row_mw")) return c->row_mw;
+    if (!strcmp(key, "col_c")) return c->col_c;
+    return ATSIM_EINVAL;
 }
 
 int atsim_device_ok(atsim_ctx* c) {
@@ -642,10 +667,10 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
             sa.lcount = s->groups[g].lcount;
             const float2* tab = s->tf + (size_t)g * W * s->N;
             long pr = prof_begin(s, PROF_ROW);
-            HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, c->stream));
+            HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, c->stream, c->row_mw));
             prof_end(s, pr);
             long pc = prof_begin(s, PROF_COL);
-            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream));
+            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream, c->col_c));
             prof_end(s, pc);
         }
     }

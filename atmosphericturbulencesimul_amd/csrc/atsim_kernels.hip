@@ -82,6 +82,17 @@ __device__ __forceinline__ double screen_phase(const ScreenArgs& sa, int d, int 
     return ph;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx b runs on
+// XCD b % 8).  xcd_swizzle maps b to a tile index so that 4 consecutive
+// tiles run on the same XCD at about the same time and share its L2 (the
+// 4 column tiles of one 128-byte line, or 4 adjacent rows that read the same
+// master rows).  Identity unless total is a multiple of 32.
+__device__ __forceinline__ int xcd_swizzle(int b, int total) {
+    if (total & 31) return b;
+    const int xcd = b & 7, r = b >> 3;
+    return ((((r >> 2) << 3) | xcd) << 2) | (r & 3);
+}
+
 // (long)(x + 1000.5) - 1000, the reference's rounding of unwrap counts
 __device__ __forceinline__ double unwrap_count(double x) { return (double)((long long)(x + 1000.5) - 1000); }
 
@@ -140,17 +151,18 @@ k_row(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
     for (int q = 0; q < 16; q++) rowp[reg_index<N, RO>(t, q)] = v[q];
 }
 
-template <int N, int OP, int DIR>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+template <int N, int OP, int DIR, int C>
+__global__ void __launch_bounds__(C * (N / 16))
 k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw) {
     using P = Plan<N>;
-    constexpr int C = ColCfg<N>::C;
     constexpr int RI = (OP == OP_FRESNEL) ? P::first(-1) : P::first(DIR);
     constexpr int RO = (OP == OP_FRESNEL) ? P::last(+1) : P::last(DIR);
     __shared__ float2 lds[C * P::LDS];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
-    const int col = blockIdx.x * C + c;
-    const int f = blockIdx.y;
+    const int ntile = N / C;
+    const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int f = tile / ntile;
+    const int col = (tile - f * ntile) * C + c;
     float2* base = data + (size_t)f * N * N + col;
     float2 v[16];
 #pragma unroll
@@ -193,7 +205,7 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     float* S = reinterpret_cast<float*>(lds);            // [4][SW] staged master rows
     double* V = reinterpret_cast<double*>(S + 4 * SW);  // [SEG] sampled screen
     const int mwc = blockDim.x / T;                      // wavelengths per workgroup (<= MW)
-    const int j = blockIdx.x, d = blockIdx.y, w0 = blockIdx.z * mwc;
+    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, w0 = blockIdx.z * mwc;
     const int nw = min(mwc, sa.W - w0);
     const int lw = threadIdx.x / T, t = threadIdx.x - lw * T;
     const bool active = lw < nw;
@@ -211,9 +223,13 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
         const double sw = sa.s[w * sa.L + l];
         for (int seg0 = 0; seg0 < N; seg0 += SEG) {
             __syncthreads();
+            const int ys = mod_m(p.y0 + j - 1, M), xst = mod_m(p.x0 + seg0 - 1, M);  // uniform
             for (int e = threadIdx.x; e < 4 * (SEG + 3); e += nthr) {
                 const int b = e / (SEG + 3), c = e - b * (SEG + 3);
-                S[b * SW + c] = m[(size_t)mod_m(p.y0 + j - 1 + b, M) * M + mod_m(p.x0 + seg0 - 1 + c, M)];
+                int y = ys + b, x = xst + c;
+                while (y >= M) y -= M;
+                while (x >= M) x -= M;
+                S[b * SW + c] = m[(size_t)y * M + x];
             }
             __syncthreads();
             for (int i = threadIdx.x; i < SEG; i += nthr) {
@@ -258,9 +274,10 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
 
 template <int N>
 static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                                       const ScreenArgs& sa, hipStream_t st) {
+                                       const ScreenArgs& sa, int mw, hipStream_t st) {
     constexpr int MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG;
-    const int mwc = sa.W < MW ? sa.W : MW;
+    int mwc = (mw > 0 && mw < MW) ? mw : MW;
+    if (sa.W < mwc) mwc = sa.W;
     size_t lds = (size_t)mwc * Plan<N>::LDS * sizeof(float2);
     const size_t staging = (size_t)16 * (SEG + 4) + (size_t)8 * SEG;
     if (lds < staging) lds = staging;
@@ -292,14 +309,40 @@ static hipError_t row_dispatch(int op, int dir, bool gen, float2* data, long F, 
     return hipGetLastError();
 }
 
+// column tile widths available for n (C * n/16 <= 1024 threads, C * LDS <= 139 KB)
+template <int N, int C>
+constexpr bool col_ok() {
+    return C <= N && C * (N / 16) <= 1024 && (long)C * Plan<N>::LDS * 8 <= 139264;
+}
+
+template <int N, int C>
+static hipError_t col_launch(int op, int dir, float2* data, long F, const float2* tf, int W, const float2* tw,
+                             hipStream_t st) {
+    if constexpr (!col_ok<N, C>()) {
+        return hipErrorInvalidValue;
+    } else {
+        dim3 grid(N / C, (unsigned)F), block(C * (N / 16));
+        if (op == OP_FRESNEL) k_col<N, OP_FRESNEL, 0, C><<<grid, block, 0, st>>>(data, tf, W, tw);
+        else if (dir < 0) k_col<N, OP_FFT, -1, C><<<grid, block, 0, st>>>(data, tf, W, tw);
+        else k_col<N, OP_FFT, +1, C><<<grid, block, 0, st>>>(data, tf, W, tw);
+        return hipGetLastError();
+    }
+}
+
 template <int N>
-static hipError_t col_dispatch(int op, int dir, float2* data, long F, const float2* tf, int W,
-                               const float2* tw, hipStream_t st) {
-    dim3 grid(N / ColCfg<N>::C, (unsigned)F), block(ColCfg<N>::THREADS);
-    if (op == OP_FRESNEL) k_col<N, OP_FRESNEL, 0><<<grid, block, 0, st>>>(data, tf, W, tw);
-    else if (dir < 0) k_col<N, OP_FFT, -1><<<grid, block, 0, st>>>(data, tf, W, tw);
-    else k_col<N, OP_FFT, +1><<<grid, block, 0, st>>>(data, tf, W, tw);
-    return hipGetLastError();
+static int col_default_c() { return ColCfg<N>::C; }
+
+template <int N>
+static hipError_t col_dispatch(int op, int dir, float2* data, long F, const float2* tf, int W, const float2* tw,
+                               int c, hipStream_t st) {
+    if (c <= 0) c = col_default_c<N>();
+    switch (c) {
+        case 2: return col_launch<N, 2>(op, dir, data, F, tf, W, tw, st);
+        case 4: return col_launch<N, 4>(op, dir, data, F, tf, W, tw, st);
+        case 8: return col_launch<N, 8>(op, dir, data, F, tf, W, tw, st);
+        case 16: return col_launch<N, 16>(op, dir, data, F, tf, W, tw, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 bool supported_n(long n) { return n >= 16 && n <= 8192 && (n & (n - 1)) == 0; }
@@ -319,10 +362,10 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 }
 
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                              const ScreenArgs& sa, hipStream_t st) {
+                              const ScreenArgs& sa, hipStream_t st, int mw) {
     switch (n) {
 #define X(NN) \
-    case NN: return row_fresnel_dispatch<NN>(gen, data, D, tf, tw, sa, st);
+    case NN: return row_fresnel_dispatch<NN>(gen, data, D, tf, tw, sa, mw, st);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;
@@ -330,10 +373,10 @@ hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float
 }
 
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
-                      const float2* tw, hipStream_t st) {
+                      const float2* tw, hipStream_t st, int c) {
     switch (n) {
 #define X(NN) \
-    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, st);
+    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, st);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;
@@ -623,6 +666,8 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     const bool blockrow = rowin && jj2 >= 0 && jj2 < xs && a.amplitude && a.bsum != nullptr;
     const int ii_lo = max(xfirst - o, 0);
     const int b0 = (xs > 0 && ii_lo < a.Nout) ? (int)((double)ii_lo / a.Nout * xs) : 0;
+    // block column of this pixel (AT.c:3611: (long)(1.0*ii/Nout*xs)); -1 if it adds to no block
+    const int ii2 = (blockrow && colin) ? (int)((double)ii / a.Nout * xs) : -1;
     const int L = a.all.L;
     const int M = (int)a.all.M;
     for (int k = threadIdx.x; k < 8 * 2 * 16; k += EXT_T) (&bs[0][0][0])[k] = 0.0;
@@ -633,9 +678,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         const LayerSample p = a.all.ls[d * L + l];
         const float* m = a.all.masters + (size_t)l * M * M;
         __syncthreads();
+        const int ys = mod_m(p.y0 + jf - 1, M), xst = mod_m(p.x0 + xfirst - 1, M);  // uniform
         for (int e = threadIdx.x; e < 4 * (EXT_T + 3); e += EXT_T) {
             const int b = e / (EXT_T + 3), c = e - b * (EXT_T + 3);
-            S[b][c] = m[(size_t)mod_m(p.y0 + jf - 1 + b, M) * M + mod_m(p.x0 + xfirst - 1 + c, M)];
+            int y = ys + b, x = xst + c;
+            while (y >= M) y -= M;
+            while (x >= M) x -= M;
+            S[b][c] = m[(size_t)y * M + x];
         }
         __syncthreads();
         if (valid) {
@@ -660,35 +709,53 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
             }
         }
     }
-    if (valid) {
+    const double inv2pi = 0.15915494309189533577;
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-            if (w >= Wc) continue;
-            const int f = d * a.W + w_begin + w;
+    for (int w = 0; w < 8; w++) {
+        if (w >= Wc) continue;
+        const int f = d * a.W + w_begin + w;
+        float cx = 0.f, cy = 0.f;  // this pixel's contribution to its block sums
+        if (valid) {
             float2 E = a.fields ? a.fields[((size_t)f * a.N + jf) * a.N + ifl] : make_float2(1.f, 0.f);
             if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
             if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
-            if (!(rowin && colin)) continue;
-            const size_t oi = ((size_t)f * a.Nout + jj) * a.Nout + ii;
-            if (!a.amplitude) {
-                a.pha[oi] = (float)PHI[w];
-                a.amp[oi] = 1.f;
-                continue;
-            }
-            const double re = E.x, im = E.y;
-            a.amp[oi] = (float)sqrt(re * re + im * im);
-            const double ph = (double)atan2f(E.y, E.x);
-            if (a.unwrap_mode[w_begin + w] == 0) {
-                a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] / 2.0 / M_PI));
-            } else {
-                a.pha[oi] = (float)ph;
-                if (a.dsave) a.dsave[oi] = PHI[w] - ph;
-                const int ii2 = (int)((double)ii / a.Nout * xs);
-                if (blockrow && ii2 < xs && ii2 - b0 < 16) {
-                    const float2 cs = expi_d(PHI[w] - ph);
-                    atomicAdd(&bs[w][0][ii2 - b0], (double)cs.x);
-                    atomicAdd(&bs[w][1][ii2 - b0], (double)cs.y);
+            if (rowin && colin) {
+                const size_t oi = ((size_t)f * a.Nout + jj) * a.Nout + ii;
+                if (!a.amplitude) {
+                    a.pha[oi] = (float)PHI[w];
+                    a.amp[oi] = 1.f;
+                } else {
+                    a.amp[oi] = sqrtf(E.x * E.x + E.y * E.y);
+                    const double ph = (double)atan2f(E.y, E.x);
+                    if (a.unwrap_mode[w_begin + w] == 0) {
+                        a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
+                    } else {
+                        a.pha[oi] = (float)ph;
+                        if (a.dsave) a.dsave[oi] = PHI[w] - ph;
+                        if (ii2 >= 0) {
+                            const float2 cs = expi_d(PHI[w] - ph);
+                            cx = cs.x;
+                            cy = cs.y;
+                        }
+                    }
                 }
+            }
+        }
+        // segmented sum over the wave's lanes (block columns are non-decreasing
+        // along the lanes); the last lane of each run adds the run's total
+        if (blockrow && a.unwrap_mode[w_begin + w] != 0) {
+            const int key = ii2;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const float ux = __shfl_up(cx, off, 64), uy = __shfl_up(cy, off, 64);
+                const int uk = __shfl_up(key, off, 64);
+                if (lane >= off && uk == key) { cx += ux; cy += uy; }
+            }
+            const int nk = __shfl_down(key, 1, 64);
+            if (key >= 0 && (lane == 63 || nk != key) && key - b0 < 16) {
+                atomicAdd(&bs[w][0][key - b0], (double)cx);
+                atomicAdd(&bs[w][1][key - b0], (double)cy);
             }
         }
     }

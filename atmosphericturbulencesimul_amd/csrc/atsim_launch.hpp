@@ -20,11 +20,13 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 // Fresnel x pass of every (direction, wavelength) field with the screen
 // phase of the layer group sa.lfirst .. + sa.lcount (sampled once per pixel
 // and direction); gen: the incoming field is 1.  Tables tf + w n.
+// mw: wavelengths per workgroup (0 = largest that fits)
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                              const ScreenArgs& sa, hipStream_t st);
+                              const ScreenArgs& sa, hipStream_t st, int mw = 0);
 // Column pass: same transforms along y; no screen.
+// c: columns per workgroup (0 = default for n)
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
-                      const float2* tw, hipStream_t st);
+                      const float2* tw, hipStream_t st, int c = 0);
 
 hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st);
 hipError_t launch_noise(float2* out, long n, uint64_t seed, uint32_t stream, hipStream_t st);

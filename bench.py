@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / FP64 error sample")
     ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--row-mw", type=int, default=0, help="wavelengths per row-pass workgroup (0 = default)")
+    ap.add_argument("--col-c", type=int, default=0, help="columns per column-pass workgroup (0 = default)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -148,6 +150,10 @@ def main():
 
     wl = headline_workload()
     ctx = Context(0)
+    if args.row_mw:
+        ctx.set_tuning("row_mw", args.row_mw)
+    if args.col_c:
+        ctx.set_tuning("col_c", args.col_c)
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -226,6 +232,7 @@ def main():
         "fft_hbm_gbs": round(fft_gbs, 1),
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
         "setup_s": round(t_setup, 3),
+        "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c")},
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
     }
     if not args.no_cpu:

@@ -44,6 +44,12 @@ int atsim_sync(atsim_ctx *ctx);
 void *atsim_stream(atsim_ctx *ctx);
 /* synchronous copy between host and/or device buffers (unified addressing) */
 int atsim_copy(atsim_ctx *ctx, void *dst, const void *src, size_t nbytes);
+/* Launch-shape tuning of this context (0 restores the built-in default):
+ *   "row_mw"  wavelength fields per row-pass workgroup (1..8)
+ *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
+ * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
+int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
+int atsim_get_tuning(atsim_ctx *ctx, const char *key);
 /* 1 if the library was built with code for the current device */
 int atsim_device_ok(atsim_ctx *ctx);
 
