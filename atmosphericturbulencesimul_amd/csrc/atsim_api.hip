@@ -480,6 +480,7 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (d->M < d->N) return fail(c, ATSIM_EINVAL, "sim: need N <= M (master %ld, grid %ld)", d->M, d->N);
     if (d->M > (1L << 20)) return fail(c, ATSIM_EINVAL, "sim: master too large");
     if (d->L <= 0 || d->W <= 0 || d->D <= 0) return fail(c, ATSIM_EINVAL, "sim: L, W, D must be > 0");
+    if (d->L > 64) return fail(c, ATSIM_EUNSUPPORTED, "sim: at most 64 layers (got %ld)", d->L);
     if (d->D * d->W > 65535) return fail(c, ATSIM_EINVAL, "sim: too many fields");
     if (!(d->dx > 0)) return fail(c, ATSIM_EINVAL, "sim: dx must be > 0");
     if (!d->slayer_alt || !d->super_layer_index || !d->lambda || !d->scoeff || !d->unwrap_mode || !d->xpos0 ||

@@ -94,10 +94,26 @@ __device__ __forceinline__ void dft16(float2* v) {
         a[n1][0] = v[n1]; a[n1][1] = v[n1 + 4]; a[n1][2] = v[n1 + 8]; a[n1][3] = v[n1 + 12];
         dft4<DIR>(a[n1][0], a[n1][1], a[n1][2], a[n1][3]);
     }
+    // internal twiddles w16^(n1 k2): m = 4 is -+i, m = 2 and 6 are (+-1 -+ i)/sqrt 2
+    constexpr float H = 0.70710678118654752f;
 #pragma unroll
     for (int n1 = 1; n1 < 4; n1++)
 #pragma unroll
-        for (int k2 = 1; k2 < 4; k2++) a[n1][k2] = cmul(a[n1][k2], w16(n1 * k2, DIR));
+        for (int k2 = 1; k2 < 4; k2++) {
+            const int m = n1 * k2;
+            float2& x = a[n1][k2];
+            if (m == 4) {
+                x = mul_qi<DIR>(x);
+            } else if (m == 2) {  // (1 + DIR i) H
+                x = DIR < 0 ? make_float2((x.x + x.y) * H, (x.y - x.x) * H)
+                            : make_float2((x.x - x.y) * H, (x.y + x.x) * H);
+            } else if (m == 6) {  // (-1 + DIR i) H
+                x = DIR < 0 ? make_float2((x.y - x.x) * H, -(x.x + x.y) * H)
+                            : make_float2(-(x.x + x.y) * H, (x.x - x.y) * H);
+            } else {
+                x = cmul(x, w16(m, DIR));
+            }
+        }
 #pragma unroll
     for (int k2 = 0; k2 < 4; k2++) {
         float2 b0 = a[0][k2], b1 = a[1][k2], b2 = a[2][k2], b3 = a[3][k2];
@@ -146,7 +162,9 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) 
     return DIR < 0 ? w : make_float2(w.x, -w.y);
 }
 
-// one stage's twiddle + butterfly on registers
+// one stage's twiddle + butterfly on registers.  The R-1 twiddles w^r of a
+// butterfly are composed from the table values w^1, w^2, w^4, w^8 (at most
+// three complex products each) instead of R-1 table loads.
 template <int N, int R, int NS, int DIR>
 __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw) {
     constexpr int T = N / 16;
@@ -156,8 +174,22 @@ __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __
             const int j = t + b * T;
             const int k = j % NS;
             constexpr int STEP = N / (NS * R);
+            float2 p[4];  // w^1, w^2, w^4, w^8
 #pragma unroll
-            for (int r = 1; r < R; r++) v[b * R + r] = cmul(v[b * R + r], twiddle<N, DIR>(tw, r * k * STEP));
+            for (int e = 0; e < 4; e++)
+                if ((1 << e) < R) p[e] = twiddle<N, DIR>(tw, (1 << e) * k * STEP);
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w = make_float2(1.f, 0.f);
+                bool first = true;
+#pragma unroll
+                for (int e = 3; e >= 0; e--)
+                    if (r & (1 << e)) {
+                        w = first ? p[e] : cmul(w, p[e]);
+                        first = false;
+                    }
+                v[b * R + r] = cmul(v[b * R + r], w);
+            }
         }
         dftR<R, DIR>(v + b * R);
     }

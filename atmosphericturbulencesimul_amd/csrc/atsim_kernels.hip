@@ -93,8 +93,61 @@ __device__ __forceinline__ int xcd_swizzle(int b, int total) {
     return ((((r >> 2) << 3) | xcd) << 2) | (r & 3);
 }
 
+// most layers a frame can hold (per-frame layer parameters are staged in LDS)
+constexpr int MAX_LAYERS = 64;
+
 // (long)(x + 1000.5) - 1000, the reference's rounding of unwrap counts
 __device__ __forceinline__ double unwrap_count(double x) { return (double)((long long)(x + 1000.5) - 1000); }
+
+// Raise a kernel's dynamic LDS limit to what the 160 KB of a gfx950 CU leaves
+// beside its static LDS.  Errors are consumed here so that they do not
+// surface from the next hipGetLastError.
+static void allow_dynamic_lds(const void* kernel) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, kernel) == hipSuccess) {
+        const int room = 160 * 1024 - (int)fa.sharedSizeBytes;
+        if (room > 0) (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, room);
+    }
+    (void)hipGetLastError();
+}
+
+
+// Copy nrows x NCOLS master values, rows ys + b and columns xst + c taken
+// modulo M (ys, xst already in [0, M)), into S[r * stride + c] where r runs
+// over the row list.  Each thread issues U loads before it writes any of them
+// to LDS, so the loads of a batch are in flight together.  Row r of the list
+// is (layer q, row b) = (r / 4, r % 4) with per-layer bases ysq[q], xsq[q].
+template <int NCOLS, int U>
+__device__ __forceinline__ void stage_master_rows(float* S, int stride, const float* __restrict__ masters,
+                                                  size_t layer_stride, int M, const int* ysq, const int* xsq,
+                                                  int l0, int nrows, int tid, int nthr) {
+    const int total = nrows * NCOLS;
+    for (int e0 = 0; e0 < total; e0 += U * nthr) {
+        float tmp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * nthr + tid;
+            if (e < total) {
+                const int r = e / NCOLS, c = e - r * NCOLS;
+                const int q = r >> 2, b = r & 3;
+                int y = ysq[q] + b;
+                y = y < M ? y : y - M;
+                int x = xsq[q] + c;
+                x = x < M ? x : x - M;
+                if (x >= M) x = x % M;
+                tmp[u] = masters[(size_t)(l0 + q) * layer_stride + (size_t)y * M + x];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * nthr + tid;
+            if (e < total) {
+                const int r = e / NCOLS;
+                S[r * stride + (e - r * NCOLS)] = tmp[u];
+            }
+        }
+    }
+}
 
 // ------------------------------------------------------------------ configs
 template <int N>
@@ -213,24 +266,31 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     const int f = d * sa.W + w;
     const int nthr = blockDim.x;
     const int M = (int)sa.M;
+    float2* rowp = data + ((size_t)f * N + j) * N;
+    float2 v[16];
+    // the field is needed only after the screen: issue its loads first
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = (GEN || !active) ? make_float2(1.f, 0.f) : rowp[reg_index<N, RI>(t, q)];
+    __shared__ LayerSample sls[MAX_LAYERS];
+    __shared__ double ss[8][MAX_LAYERS];
+    for (int k = threadIdx.x; k < sa.lcount; k += nthr) sls[k] = sa.ls[d * sa.L + sa.lfirst + k];
+    for (int k = threadIdx.x; k < mwc * sa.lcount; k += nthr) {
+        const int lw2 = k / sa.lcount, l = k - lw2 * sa.lcount;
+        ss[lw2][l] = sa.s[min(w0 + lw2, sa.W - 1) * sa.L + sa.lfirst + l];
+    }
     double ph[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) ph[q] = 0.0;
     for (int k = 0; k < sa.lcount; k++) {
         const int l = sa.lfirst + k;
-        const LayerSample p = sa.ls[d * sa.L + l];
+        __syncthreads();
+        const LayerSample& p = sls[k];
         const float* m = sa.masters + (size_t)l * M * M;
-        const double sw = sa.s[w * sa.L + l];
+        const double sw = ss[lw][k];
         for (int seg0 = 0; seg0 < N; seg0 += SEG) {
             __syncthreads();
             const int ys = mod_m(p.y0 + j - 1, M), xst = mod_m(p.x0 + seg0 - 1, M);  // uniform
-            for (int e = threadIdx.x; e < 4 * (SEG + 3); e += nthr) {
-                const int b = e / (SEG + 3), c = e - b * (SEG + 3);
-                int y = ys + b, x = xst + c;
-                while (y >= M) y -= M;
-                while (x >= M) x -= M;
-                S[b * SW + c] = m[(size_t)y * M + x];
-            }
+            stage_master_rows<SEG + 3, 9>(S, SW, m, 0, M, &ys, &xst, 0, 4, threadIdx.x, nthr);
             __syncthreads();
             for (int i = threadIdx.x; i < SEG; i += nthr) {
                 double v = 0.0;
@@ -253,13 +313,8 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
             }
         }
     }
-    float2* rowp = data + ((size_t)f * N + j) * N;
-    float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const float2 e = expi_d(ph[q]);
-        v[q] = GEN ? e : (active ? cmul(rowp[reg_index<N, RI>(t, q)], e) : e);
-    }
+    for (int q = 0; q < 16; q++) v[q] = cmul(v[q], expi_d(ph[q]));
     float2* Lr = lds + lw * P::LDS;
     fft_regs<N, -1, 1>(v, Lr, t, 0, tw);
     const float2* h = tf + (size_t)w * N;
@@ -284,9 +339,8 @@ static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const floa
     dim3 grid(N, D, (sa.W + mwc - 1) / mwc), block(mwc * (N / 16));
     static bool attr = false;
     if (!attr) {
-        const int mx = (int)(160 * 1024);
-        (void)hipFuncSetAttribute((const void*)k_row_fresnel<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        (void)hipFuncSetAttribute((const void*)k_row_fresnel<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        allow_dynamic_lds((const void*)k_row_fresnel<N, true>);
+        allow_dynamic_lds((const void*)k_row_fresnel<N, false>);
         attr = true;
     }
     if (gen) k_row_fresnel<N, true><<<grid, block, lds, st>>>(data, tf, tw, sa);
@@ -640,10 +694,13 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 // by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l and the
 // screen of the last layer group, applied here after the last Fresnel gap.
 constexpr int EXT_T = 256;
+constexpr int EXT_LC = 10;  // layers staged per LDS fill
 template <bool FULL>
 __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
-    __shared__ float S[4][EXT_T + 4];
+    __shared__ float S[EXT_LC][4][EXT_T + 4];
     __shared__ double bs[8][2][16];
+    __shared__ LayerSample sls[MAX_LAYERS];  // this direction's layer samples
+    __shared__ double ss[8][MAX_LAYERS];     // phase scales of this chunk's wavelengths
     const int nchunks = (a.W + 7) / 8;
     const int seg = blockIdx.x, row = blockIdx.y;
     const int d = blockIdx.z / nchunks, chunk = blockIdx.z - d * nchunks;
@@ -670,41 +727,58 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     const int ii2 = (blockrow && colin) ? (int)((double)ii / a.Nout * xs) : -1;
     const int L = a.all.L;
     const int M = (int)a.all.M;
+    // per-frame layer parameters, read once into LDS (global loads of uniform
+    // values would otherwise be issued per lane inside the layer loop)
+    __shared__ int ysq[MAX_LAYERS], xsq[MAX_LAYERS];  // wrapped first staged row / column per layer
+    for (int k = threadIdx.x; k < L; k += EXT_T) {
+        const LayerSample p = a.all.ls[d * L + k];
+        sls[k] = p;
+        ysq[k] = mod_m(p.y0 + jf - 1, M);
+        xsq[k] = mod_m(p.x0 + xfirst - 1, M);
+    }
+    for (int k = threadIdx.x; k < 8 * L; k += EXT_T) {
+        const int w = k / L, l = k - w * L;
+        ss[w][l] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
+    }
+    __syncthreads();
+    // the fields are needed only after the layers: issue their loads first
+    float2 Ein[8];
+#pragma unroll
+    for (int w = 0; w < 8; w++)
+        Ein[w] = (valid && w < Wc && a.fields) ? a.fields[((size_t)(d * a.W + w_begin + w) * a.N + jf) * a.N + ifl]
+                                               : make_float2(1.f, 0.f);
     for (int k = threadIdx.x; k < 8 * 2 * 16; k += EXT_T) (&bs[0][0][0])[k] = 0.0;
     double PHI[8], PL[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
-    for (int l = 0; l < L; l++) {
-        const LayerSample p = a.all.ls[d * L + l];
-        const float* m = a.all.masters + (size_t)l * M * M;
+    for (int l0 = 0; l0 < L; l0 += EXT_LC) {
+        const int nl = min(EXT_LC, L - l0);
         __syncthreads();
-        const int ys = mod_m(p.y0 + jf - 1, M), xst = mod_m(p.x0 + xfirst - 1, M);  // uniform
-        for (int e = threadIdx.x; e < 4 * (EXT_T + 3); e += EXT_T) {
-            const int b = e / (EXT_T + 3), c = e - b * (EXT_T + 3);
-            int y = ys + b, x = xst + c;
-            while (y >= M) y -= M;
-            while (x >= M) x -= M;
-            S[b][c] = m[(size_t)y * M + x];
-        }
+        stage_master_rows<EXT_T + 3, 21>(&S[0][0][0], EXT_T + 4, a.all.masters, (size_t)M * M, M, &ysq[l0],
+                                         &xsq[l0], l0, nl * 4, threadIdx.x, EXT_T);
         __syncthreads();
         if (valid) {
-            double v = 0.0;
+            for (int q = 0; q < nl; q++) {
+                const int l = l0 + q;
+                const LayerSample& p = sls[l];
+                double v = 0.0;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const float* r = &S[b][threadIdx.x];
-                double h = p.wx[0] * (double)r[0];
-                h = fma(p.wx[1], (double)r[1], h);
-                h = fma(p.wx[2], (double)r[2], h);
-                h = fma(p.wx[3], (double)r[3], h);
-                v = fma(p.wy[b], h, v);
-            }
-            const bool inlast = l >= a.last_first && l < a.last_first + a.last_count;
+                for (int b = 0; b < 4; b++) {
+                    const float* r = &S[q][b][threadIdx.x];
+                    double h = p.wx[0] * (double)r[0];
+                    h = fma(p.wx[1], (double)r[1], h);
+                    h = fma(p.wx[2], (double)r[2], h);
+                    h = fma(p.wx[3], (double)r[3], h);
+                    v = fma(p.wy[b], h, v);
+                }
+                const bool inlast = l >= a.last_first && l < a.last_first + a.last_count;
 #pragma unroll
-            for (int w = 0; w < 8; w++) {
-                if (w < Wc) {
-                    const double sv = a.all.s[(w_begin + w) * L + l] * v;
-                    PHI[w] += sv;
-                    if (inlast) PL[w] += sv;
+                for (int w = 0; w < 8; w++) {
+                    if (w < Wc) {
+                        const double sv = ss[w][l] * v;
+                        PHI[w] += sv;
+                        if (inlast) PL[w] += sv;
+                    }
                 }
             }
         }
@@ -717,7 +791,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         const int f = d * a.W + w_begin + w;
         float cx = 0.f, cy = 0.f;  // this pixel's contribution to its block sums
         if (valid) {
-            float2 E = a.fields ? a.fields[((size_t)f * a.N + jf) * a.N + ifl] : make_float2(1.f, 0.f);
+            float2 E = Ein[w];
             if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
             if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
             if (rowin && colin) {
@@ -734,9 +808,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                         a.pha[oi] = (float)ph;
                         if (a.dsave) a.dsave[oi] = PHI[w] - ph;
                         if (ii2 >= 0) {
-                            const float2 cs = expi_d(PHI[w] - ph);
-                            cx = cs.x;
-                            cy = cs.y;
+                            // contribution to the block's mean direction: FP64
+                            // reduction to [-pi, pi], hardware sin/cos (the
+                            // block sums only feed the atan2 of a 400-pixel sum)
+                            const double dd = PHI[w] - ph;
+                            const float r = (float)(dd - 6.283185307179586477 * rint(dd * inv2pi));
+                            cx = __cosf(r);
+                            cy = __sinf(r);
                         }
                     }
                 }
@@ -869,7 +947,7 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* fi
     const int use_lds = need <= 150 * 1024;
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void*)k_block_fix, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        allow_dynamic_lds((const void*)k_block_fix);
         attr = true;
     }
     k_block_fix<<<nfields, 1024, use_lds ? need : 0, st>>>(bsum, bin, ch, field_list, xs, use_lds);

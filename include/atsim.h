@@ -120,7 +120,7 @@ typedef struct {
     long N;        /* WF_RAW_SIZE: propagation grid (power of 2) */
     long Nout;     /* WFsize: centred output crop, Nout <= N */
     long M;        /* MASTER_SIZE: master screen size, N <= M */
-    long L;        /* layers (ascending altitude) */
+    long L;        /* layers (ascending altitude), at most 64 */
     long W;        /* wavelengths (fields per direction) */
     long D;        /* directions */
     double dx;     /* PUPIL_SCALE [m/pixel] */
