@@ -40,6 +40,8 @@ _SIGS = {
     "atsim_stream": (C.c_void_p, [C.c_void_p]),
     "atsim_device_ok": (C.c_int, [C.c_void_p]),
     "atsim_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "atsim_malloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t]),
+    "atsim_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "atsim_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "atsim_fft2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int]),
@@ -65,6 +67,7 @@ _SIGS = {
     "atsim_sim_set_master": (C.c_int, [C.c_void_p, C.c_long, C.c_void_p]),
     "atsim_sim_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "atsim_sim_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "atsim_sim_set_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "atsim_sim_gaps": (C.c_long, [C.c_void_p]),
     "atsim_sim_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "atsim_sim_synth_masters": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
     "-I", INCLUDE, "-I", CSRC,
 ]
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-I", INCLUDE, "-I", HOST]
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-format-truncation", "-I", INCLUDE, "-I", HOST]
 
 
 def _sources():

@@ -113,6 +113,26 @@ int atsim_sync(atsim_ctx* c) {
 
 void* atsim_stream(atsim_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int atsim_malloc(atsim_ctx* c, void** ptr, size_t nbytes) {
+    if (!c || !ptr) return ATSIM_EINVAL;
+    *ptr = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (hipMalloc(ptr, nbytes > 0 ? nbytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, ATSIM_ENOMEM, "cannot allocate %zu device bytes", nbytes);
+    }
+    return ATSIM_OK;
+}
+
+int atsim_free(atsim_ctx* c, void* ptr) {
+    if (!c) return ATSIM_EINVAL;
+    if (ptr) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(ptr));
+    }
+    return ATSIM_OK;
+}
+
 int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
     if (!c || !dst || !src) return ATSIM_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -595,6 +615,13 @@ int atsim_sim_set_master(atsim_sim* s, long layer, const float* src) {
     atsim_ctx* c = s->ctx;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(s->masters + (size_t)layer * s->M * s->M, src, sizeof(float) * s->M * s->M, hipMemcpyDefault));
+    return ATSIM_OK;
+}
+
+int atsim_sim_set_positions(atsim_sim* s, const double* xp, const double* yp) {
+    if (!s || !xp || !yp) return ATSIM_EINVAL;
+    memcpy(s->xpos.data(), xp, sizeof(double) * s->xpos.size());
+    memcpy(s->ypos.data(), yp, sizeof(double) * s->ypos.size());
     return ATSIM_OK;
 }
 

@@ -42,6 +42,9 @@ const char *atsim_error(const atsim_ctx *ctx);
 int atsim_sync(atsim_ctx *ctx);
 /* HIP stream (hipStream_t) all work of this context is queued on */
 void *atsim_stream(atsim_ctx *ctx);
+/* device memory of the context's device */
+int atsim_malloc(atsim_ctx *ctx, void **ptr, size_t nbytes);
+int atsim_free(atsim_ctx *ctx, void *ptr);
 /* synchronous copy between host and/or device buffers (unified addressing) */
 int atsim_copy(atsim_ctx *ctx, void *dst, const void *src, size_t nbytes);
 /* Launch-shape tuning of this context (0 restores the built-in default):
@@ -163,6 +166,10 @@ int atsim_sim_normalize_masters(atsim_sim *sim, double r0_goal_px, const double 
  * [D][W][Nout][Nout] (phase in rad, unwrapped; amplitude); d_field (nullable)
  * receives the final complex fields [D][W][N][N]. */
 int atsim_sim_frame(atsim_sim *sim, float *d_pha, float *d_amp, void *d_field);
+/* Replace the current FP64 positions [D][L] [pixels]; the next frame
+ * advances from them by v dt (the reference's REALTIME mode sets positions
+ * from the clock, AT.c:2700-2704). */
+int atsim_sim_set_positions(atsim_sim *sim, const double *xpos, const double *ypos);
 /* current FP64 positions [D][L] (host arrays, nullable) */
 int atsim_sim_positions(atsim_sim *sim, double *xpos, double *ypos);
 /* number of Fresnel gaps per frame (row + column pass each) */
