@@ -137,6 +137,9 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--row-mw", type=int, default=0, help="wavelengths per row-pass workgroup (0 = default)")
     ap.add_argument("--col-c", type=int, default=0, help="columns per column-pass workgroup (0 = default)")
+    ap.add_argument("--host-copy-steps", type=int, default=20,
+                    help="extra frames each followed by a synchronous copy of OPD + amplitude to pinned host "
+                         "memory (the PCIe-inclusive rate of a host hand-over; 0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -161,7 +164,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    sim, _ = make_sim(ctx, wl)
+    sim, setup = make_sim(ctx, wl)
     t0 = time.perf_counter()
     sim.synth_masters(wl["seed"], wl["L0"] / wl["dx"], wl["l0"] / wl["dx"])
     r0_meas = sim.normalize_masters(wl["r0"] / wl["dx"], wl["cn2"])
@@ -183,17 +186,44 @@ def main():
     elapsed = time.perf_counter() - t0
     prof = {k: sim.profile_read(k) for k in ("row", "col", "extract", "block")}
     sim.profile(False)
+    pcie = None
+    if args.host_copy_steps > 0:
+        from atmosphericturbulencesimul_amd.api import hip_copy
+        hp = torch.empty(tuple(sim.pha.shape), dtype=torch.float32, pin_memory=True)
+        ha = torch.empty(tuple(sim.amp.shape), dtype=torch.float32, pin_memory=True)
+        nb = sim.pha.numel() * 4
+        t0 = time.perf_counter()
+        for _ in range(args.host_copy_steps):
+            pha_d, amp_d = sim.frame(sync=False)
+            hip_copy(ctx, hp.data_ptr(), pha_d.data_ptr(), nb)
+            hip_copy(ctx, ha.data_ptr(), amp_d.data_ptr(), nb)
+        el = time.perf_counter() - t0
+        pcie = {"value": round(args.host_copy_steps * F / el, 2), "unit": "wavefront-frames/s",
+                "steps": args.host_copy_steps, "bytes_to_host_per_step": 2 * nb,
+                "how": "each frame followed by a synchronous copy of its OPD and amplitude maps to pinned host memory"}
 
     N = wl["N"]
-    pass_bytes = 16.0 * N * N * F  # one read + one write of every field per pass
+    # algorithmic bytes per launch.  Column pass: one c64 read + write of every
+    # field.  Row pass of gap g: the same, except that the top group's pass
+    # creates the field (write only), plus one f32 read of the footprint of
+    # each layer of group g per direction (the last group is applied by the
+    # extraction).  Averaged over the G launches of a frame.
+    sli = np.asarray(setup["sli"])
+    lcount_last = int((sli == sli[0]).sum())  # the ground group holds layer 0
+    col_bytes = 16.0 * N * N * F
+    row_bytes = ((G - 1) * 16.0 + 8.0) * N * N * F / G + 4.0 * N * N * wl["D"] * (wl["L"] - lcount_last) / G
     avg = {k: (ms / n if n else float("nan")) for k, (ms, n) in prof.items()}
     dom = "row" if prof["row"][0] >= prof["col"][0] else "col"
-    achieved = pass_bytes / (avg[dom] * 1e-3) / 1e9
+    dom_bytes = row_bytes if dom == "row" else col_bytes
+    achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
     fft_ms = prof["row"][0] + prof["col"][0]
-    fft_gbs = pass_bytes * (prof["row"][1] + prof["col"][1]) / (fft_ms * 1e-3) / 1e9
+    fft_gbs = (row_bytes * prof["row"][1] + col_bytes * prof["col"][1]) / (fft_ms * 1e-3) / 1e9
     value = args.steps * F / elapsed
-    kernel_names = {"row": "k_row<2048, OP_FRESNEL> (x pass: screen phase, FFT, transfer function, IFFT)",
-                    "col": "k_col<2048, OP_FRESNEL> (y pass: FFT, transfer function, IFFT)"}
+    # whole-frame algorithmic bytes per wavefront-frame: two c64 passes per gap,
+    # one f32 read of each layer's footprint, an f32 OPD and amplitude write
+    frame_bytes = 32.0 * G * N * N + 4.0 * wl["L"] * N * N + 8.0 * wl["Nout"] ** 2
+    kernel_names = {"row": "k_row_fresnel<2048> (x pass: screen phase, FFT, transfer function, IFFT)",
+                    "col": "k_col<2048> (y pass: FFT, transfer function, IFFT)"}
     traffic = None
     tr = read_traffic()
     if tr and dom in tr:
@@ -226,11 +256,15 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "bytes_per_launch": pass_bytes,
+            "bytes_per_launch": dom_bytes,
             "avg_launch_ms": round(avg[dom], 5),
         },
         "fft_hbm_gbs": round(fft_gbs, 1),
+        "frame_roofline": {"bytes_per_wavefront_frame": frame_bytes,
+                           "achieved_gbs": round(frame_bytes * value / 1e9, 1),
+                           "frac": round(frame_bytes * value / 1e9 / HBM_PEAK_GBS, 4)},
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "pcie_inclusive": pcie,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c")},
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
