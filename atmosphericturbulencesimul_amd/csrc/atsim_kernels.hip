@@ -60,15 +60,34 @@ __device__ __forceinline__ double sample_layer(const float* __restrict__ m, int 
 }
 
 // e^{i ph}: argument reduced modulo 2 pi in FP64, sin/cos in FP32
+// e^{i ph} for an FP64 phase.  The phase is reduced in FP64 to a quarter
+// turn q and a remainder t = ph/2pi - q/4 in [-1/8, 1/8] revolutions (so the
+// float remainder carries ~2e-8 rad of rounding, not the ~2e-7 of an angle
+// near pi); sin(2 pi t) and cos(2 pi t) are their Taylor series through
+// t^9 / t^10 (truncation below 3e-8 on |2 pi t| <= pi/4), evaluated in
+// FP32 Horner form, then rotated by q quarter turns.
 __device__ __forceinline__ float2 expi_d(double ph) {
-    const double inv2pi = 0.15915494309189533577;
-    const double twopi_hi = 6.28318530717958623200, twopi_lo = 2.44929359829470635445e-16;
-    double k = rint(ph * inv2pi);
-    double r = fma(-k, twopi_hi, ph);
-    r = fma(-k, twopi_lo, r);
-    float s, c;
-    sincosf((float)r, &s, &c);
-    return make_float2(c, s);
+    const double rr = ph * 0.15915494309189533577;  // revolutions
+    const double q4 = rint(rr * 4.0);
+    const float t = (float)fma(q4, -0.25, rr);
+    const int qi = (int)q4 & 3;  // quarter turns mod 4 (two's complement)
+    const float t2 = t * t;
+    float s = fmaf(t2, 42.058693944897634f, -76.70585975306136f);
+    s = fmaf(t2, s, 81.60524927607504f);
+    s = fmaf(t2, s, -41.341702240399755f);
+    s = fmaf(t2, s, 6.283185307179586f);
+    s *= t;
+    float c = fmaf(t2, -26.426256783374388f, 60.24464137187664f);
+    c = fmaf(t2, c, -85.45681720669371f);
+    c = fmaf(t2, c, 64.93939402266828f);
+    c = fmaf(t2, c, -19.739208802178716f);
+    c = fmaf(t2, c, 1.0f);
+    // (cos, sin)(a + qi pi/2)
+    float sr = (qi & 1) ? c : s;
+    float cr = (qi & 1) ? s : c;
+    if (qi & 2) sr = -sr;
+    if ((qi + 1) & 2) cr = -cr;
+    return make_float2(cr, sr);
 }
 
 __device__ __forceinline__ double screen_phase(const ScreenArgs& sa, int d, int w, int i, int j) {
@@ -278,9 +297,8 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
         const int lw2 = k / sa.lcount, l = k - lw2 * sa.lcount;
         ss[lw2][l] = sa.s[min(w0 + lw2, sa.W - 1) * sa.L + sa.lfirst + l];
     }
-    double ph[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) ph[q] = 0.0;
+    // each layer of the group multiplies the field by its own e^{i s v}: no
+    // per-element phase accumulators stay live across the staging
     for (int k = 0; k < sa.lcount; k++) {
         const int l = sa.lfirst + k;
         __syncthreads();
@@ -309,12 +327,10 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int idx = reg_index<N, RI>(t, q) - seg0;
-                if (idx >= 0 && idx < SEG) ph[q] = fma(sw, V[idx], ph[q]);
+                if (SEG == N || (idx >= 0 && idx < SEG)) v[q] = cmul(v[q], expi_d(sw * V[idx]));
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = cmul(v[q], expi_d(ph[q]));
     float2* Lr = lds + lw * P::LDS;
     fft_regs<N, -1, 1>(v, Lr, t, 0, tw);
     const float2* h = tf + (size_t)w * N;
