@@ -26,6 +26,7 @@ struct atsim_ctx {
     // launch-shape tuning (0 = built-in default)
     int row_mw = 0;  // wavelengths per row-pass workgroup
     int col_c = 0;   // columns per column-pass workgroup
+    double* nn_part = nullptr;  // 256 partial sums of the network's loss reduction
 };
 
 static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
@@ -99,6 +100,7 @@ void atsim_ctx_destroy(atsim_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (auto& kv : c->twiddles) hipFree(kv.second);
+    if (c->nn_part) hipFree(c->nn_part);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -807,6 +809,73 @@ int atsim_sim_profile_read(atsim_sim* s, int kind, double* ms, long* count) {
     }
     if (ms) *ms = tot;
     if (count) *count = n;
+    return ATSIM_OK;
+}
+
+// ------------------------------------------------------------------ layer screens
+int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {
+    if (!s || !d_out) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    HIPCHK(c, launch_layer_screens(s->masters, s->ls_dev, d_out, s->N, s->M, s->L, s->D, c->stream));
+    return ATSIM_OK;
+}
+
+// ------------------------------------------------------------------ surrogate network
+int atsim_gemm_bf16(atsim_ctx* c, long M, long N, long K, long batch, float alpha, const float* A, long sab,
+                    long sam, long sak, const float* B, long sbb, long sbk, long sbn, float beta, float* C, long scb,
+                    long scm, long scn) {
+    if (!c || !A || !B || !C || M < 0 || N < 0 || K < 0 || batch < 0) return ATSIM_EINVAL;
+    GemmArgs g{M, N, K, batch, alpha, beta, A, sab, sam, sak, B, sbb, sbk, sbn, C, scb, scm, scn};
+    hipError_t e = launch_gemm_bf16(g, c->stream);
+    if (e == hipErrorInvalidValue) return fail(c, ATSIM_EUNSUPPORTED, "gemm: grid too large (M %ld, batch %ld)", M, batch);
+    HIPCHK(c, e);
+    return ATSIM_OK;
+}
+
+int atsim_nn_add_bias_act(atsim_ctx* c, const float* a, const float* b, const float* bias, float* pre, float* y,
+                          long C, long P, long n, int gelu) {
+    if (!c || !a || !y || C <= 0 || P <= 0 || n < 0 || n % (C * P)) return ATSIM_EINVAL;
+    HIPCHK(c, launch_add_bias_act(a, b, bias, pre, y, C, P, n, gelu, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_gelu_bwd(atsim_ctx* c, const float* pre, const float* dy, float* dx, long n) {
+    if (!c || !pre || !dy || !dx || n < 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_gelu_bwd(pre, dy, dx, n, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_bias_grad(atsim_ctx* c, const float* dy, float* db, long S, long C, long P, int accumulate) {
+    if (!c || !dy || !db || S <= 0 || C <= 0 || P <= 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_bias_grad(dy, db, S, C, P, accumulate, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_mse(atsim_ctx* c, const float* pred, const float* target, float* grad, long n, double* loss) {
+    if (!c || !pred || !target || n <= 0) return ATSIM_EINVAL;
+    const int nblk = 256;
+    if (!c->nn_part) HIPCHK(c, hipMalloc(&c->nn_part, sizeof(double) * nblk));
+    HIPCHK(c, launch_mse(pred, target, grad, n, c->nn_part, nblk, c->stream));
+    if (loss) {
+        std::vector<double> v;
+        int r = reduce_partials(c, c->nn_part, nblk, 1, v);
+        if (r) return r;
+        *loss = v[0] / (double)n;
+    }
+    return ATSIM_OK;
+}
+
+int atsim_nn_reduce_batches(atsim_ctx* c, const float* part, float* out, long nb, long n, int accumulate) {
+    if (!c || !part || !out || nb <= 0 || n <= 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_reduce_batches(part, out, nb, n, accumulate, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_adam(atsim_ctx* c, float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2,
+                  float eps, long step) {
+    if (!c || !w || !g || !m || !v || n < 0 || step < 1) return ATSIM_EINVAL;
+    const float c1 = (float)(1.0 - pow((double)b1, (double)step)), c2 = (float)(1.0 - pow((double)b2, (double)step));
+    HIPCHK(c, launch_adam(w, g, m, v, n, lr, b1, b2, eps, c1, c2, c->stream));
     return ATSIM_OK;
 }
 

@@ -59,7 +59,6 @@ __device__ __forceinline__ double sample_layer(const float* __restrict__ m, int 
     return v;
 }
 
-// e^{i ph}: argument reduced modulo 2 pi in FP64, sin/cos in FP32
 // e^{i ph} for an FP64 phase.  The phase is reduced in FP64 to a quarter
 // turn q and a remainder t = ph/2pi - q/4 in [-1/8, 1/8] revolutions (so the
 // float remainder carries ~2e-8 rad of rounding, not the ~2e-7 of an angle
@@ -995,6 +994,30 @@ hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin
     int bx = (int)((no + 255) / 256 < 4096 ? (no + 255) / 256 : 4096);
     dim3 grid(bx, nfields);
     k_block_apply<<<grid, 256, 0, st>>>(pha, dsave, bin, field_list, Nout, xs);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ layer screens
+// out[(d L + l) N^2 + j N + i] = sampled screen of layer l (FP64 sampler,
+// rounded to float) at the positions of the last frame: the per-layer phase
+// stack the surrogate network reads.
+__global__ void k_layer_screens(const float* __restrict__ masters, const LayerSample* __restrict__ ls,
+                                float* __restrict__ out, int N, int M, int L) {
+    const int l = blockIdx.y, d = blockIdx.z;
+    const long n = (long)N * N;
+    const LayerSample p = ls[d * L + l];
+    const float* m = masters + (size_t)l * M * M;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(e / N), i = (int)(e - (long)j * N);
+        out[((long)d * L + l) * n + e] = (float)sample_layer(m, M, p, i, j);
+    }
+}
+
+hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, float* out, long N, long M, long L,
+                                long D, hipStream_t st) {
+    long g = (N * N + 255) / 256;
+    dim3 grid((unsigned)(g < 4096 ? g : 4096), (unsigned)L, (unsigned)D);
+    k_layer_screens<<<grid, 256, 0, st>>>(masters, ls, out, (int)N, (int)M, (int)L);
     return hipGetLastError();
 }
 

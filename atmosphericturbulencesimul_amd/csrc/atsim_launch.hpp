@@ -50,4 +50,29 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* fi
 hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
                               int nfields, int Nout, int xs, hipStream_t st);
 
+hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, float* out, long N, long M, long L,
+                                long D, hipStream_t st);
+
+// surrogate network (atsim_fno.hip)
+struct GemmArgs {
+    long M, N, K, batch;
+    float alpha, beta;
+    const float* A;
+    long sab, sam, sak;
+    const float* B;
+    long sbb, sbk, sbn;
+    float* C;
+    long scb, scm, scn;
+};
+hipError_t launch_gemm_bf16(const GemmArgs& g, hipStream_t st);
+hipError_t launch_add_bias_act(const float* a, const float* b, const float* bias, float* pre, float* y, long C,
+                               long P, long n, int gelu, hipStream_t st);
+hipError_t launch_gelu_bwd(const float* pre, const float* dy, float* dx, long n, hipStream_t st);
+hipError_t launch_bias_grad(const float* dy, float* db, long S, long C, long P, int accumulate, hipStream_t st);
+hipError_t launch_mse(const float* pred, const float* target, float* grad, long n, double* partial, int nblk,
+                      hipStream_t st);
+hipError_t launch_reduce_batches(const float* part, float* out, long nb, long n, int accumulate, hipStream_t st);
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t st);
+
 }  // namespace atsim

@@ -13,8 +13,10 @@ wavefront-frames.
 Printed JSON line: value = wavefront-frames/s with everything resident in
 HBM; roofline of the dominant kernel from HIP events recorded around each of
 its launches inside the timed region; cpu_baseline = the CPU oracle on a
-bounded sample of the same workload (single core), which also gives the
-FP64 error of one full-size field.
+bounded sample of the same workload (one wavelength per host thread), which
+also gives the FP64 error of every full-size field; pcie_inclusive = the rate
+with each frame handed to pinned host memory; per_wavelength_plain = the
+same frames from one simulation per wavelength.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -157,6 +159,8 @@ def main():
     ap.add_argument("--host-copy-steps", type=int, default=20,
                     help="extra frames each followed by a synchronous copy of OPD + amplitude to pinned host "
                          "memory (the PCIe-inclusive rate of a host hand-over; 0 = skip)")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="frames of the per-wavelength comparison (one simulation per wavelength; 0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -218,6 +222,30 @@ def main():
         pcie = {"value": round(args.host_copy_steps * F / el, 2), "unit": "wavefront-frames/s",
                 "steps": args.host_copy_steps, "bytes_to_host_per_step": 2 * nb,
                 "how": "each frame followed by a synchronous copy of its OPD and amplitude maps to pinned host memory"}
+    per_lambda = None
+    if args.plain_steps > 0:
+        # the same wavefront-frames from one simulation per wavelength (one
+        # field per launch), the reference's per-wavelength structure
+        sims1 = []
+        for w in range(wl["W"]):
+            s1, _ = make_sim(ctx, wl, lambdas=wl["lambdas"][w:w + 1], unwrap=[int(setup["unwrap"][w])])
+            for l in range(wl["L"]):
+                s1.set_master(l, sim.master(l))
+            sims1.append(s1)
+        for s1 in sims1:
+            s1.frame(sync=False)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            for s1 in sims1:
+                s1.frame(sync=False)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        for s1 in sims1:
+            s1.close()
+        v1 = args.plain_steps * wl["W"] / el
+        per_lambda = {"value": round(v1, 2), "unit": "wavefront-frames/s", "steps": args.plain_steps,
+                      "how": "one simulation per wavelength, one field per launch, same masters and kernels"}
 
     N = wl["N"]
     # algorithmic bytes per launch.  Column pass: one c64 read + write of every
@@ -282,6 +310,8 @@ def main():
                            "frac": round(frame_bytes * value / 1e9 / HBM_PEAK_GBS, 4)},
         "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
         "pcie_inclusive": pcie,
+        "per_wavelength_plain": per_lambda,
+        "batched_vs_per_wavelength": round(value / per_lambda["value"], 2) if per_lambda else None,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c")},
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],

@@ -181,6 +181,43 @@ long atsim_sim_gaps(atsim_sim *sim);
 int atsim_sim_profile(atsim_sim *sim, int enable);
 int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
 
+/* Per-layer sampled screens at the positions of the last frame (the FP64
+ * sampler of the frame loop, rounded to float): d_out [D][L][N][N], radians
+ * at the reference wavelength before Scoeff.  No counterpart in the
+ * reference: the input of the surrogate network. */
+int atsim_sim_layer_screens(atsim_sim *sim, float *d_out);
+
+/* ---------------------------------------------------------------- surrogate network
+ * Building blocks of the Fourier-neural-operator surrogate of the layer loop
+ * (no counterpart in the reference).  All buffers are device FP32.
+ *
+ * atsim_gemm_bf16: for b < batch, C_b = alpha A_b B_b + beta C_b with
+ * A_b (M x K) at A + b sab + m sam + k sak, B_b (K x N) at B + b sbb + k sbk
+ * + n sbn, C_b (M x N) at C + b scb + m scm + n scn (element strides).  The
+ * operands are rounded to BF16 and multiplied on the matrix cores with FP32
+ * accumulation.  batch and ceil(M / 64) at most 65535. */
+int atsim_gemm_bf16(atsim_ctx *ctx, long M, long N, long K, long batch, float alpha,
+                    const float *A, long sab, long sam, long sak,
+                    const float *B, long sbb, long sbk, long sbn, float beta,
+                    float *C, long scb, long scm, long scn);
+/* y = act(a + b + bias[c]) over n = S*C*P elements laid out [S][C][P]; b, bias
+ * and pre may be NULL; pre keeps the activation's input; act = GELU (tanh
+ * form) if gelu != 0, else identity. */
+int atsim_nn_add_bias_act(atsim_ctx *ctx, const float *a, const float *b, const float *bias, float *pre,
+                          float *y, long C, long P, long n, int gelu);
+/* dx = dy * GELU'(pre) */
+int atsim_nn_gelu_bwd(atsim_ctx *ctx, const float *pre, const float *dy, float *dx, long n);
+/* db[c] = sum_{s,p} dy[s][c][p] (added to db if accumulate) */
+int atsim_nn_bias_grad(atsim_ctx *ctx, const float *dy, float *db, long S, long C, long P, int accumulate);
+/* mean squared error; grad (may be NULL) = 2 (pred - target) / n; loss (may
+ * be NULL; synchronises the stream) = mean of (pred - target)^2 */
+int atsim_nn_mse(atsim_ctx *ctx, const float *pred, const float *target, float *grad, long n, double *loss);
+/* out[i] = sum_{b < nb} part[b n + i] (added to out if accumulate) */
+int atsim_nn_reduce_batches(atsim_ctx *ctx, const float *part, float *out, long nb, long n, int accumulate);
+/* one Adam step (bias-corrected, step >= 1) on n parameters */
+int atsim_nn_adam(atsim_ctx *ctx, float *w, const float *g, float *m, float *v, long n, float lr, float b1,
+                  float b2, float eps, long step);
+
 #ifdef __cplusplus
 }
 #endif

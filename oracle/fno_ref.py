@@ -1,0 +1,124 @@
+"""FP64 numpy restatement of the Fourier-neural-operator surrogate.
+
+TEST INFRASTRUCTURE ONLY: the checker of the GPU network in
+atmosphericturbulencesimul_amd/fno.py (forward values and every gradient).
+The surrogate has no counterpart in the reference.
+
+Network (all activations channel-major [C][S*P], P = N*N pixels of S samples):
+    h0  = Lw x + Lb                                  lift, cin -> width
+    h_{b+1} = act_b(Wb h_b + Bb + K_b h_b)           b < blocks; act = GELU
+                                                     except for the last block
+    K_b h = Re IDFT2( R_b(k) . DFT2(h)(k) )          retained modes
+            k = (ky, kx), ky in [-m, m), kx in [0, m), per mode a complex
+            width x width matrix; the inverse keeps kx = 0 once and
+            doubles kx > 0 (the real-input convention), scaled by 1/P
+    y   = Q2 GELU(Q1 h + q1b) + q2b                  projection to cout
+Loss: mean squared error over all outputs.
+
+GELU is the tanh form 0.5 x (1 + tanh(sqrt(2/pi)(x + 0.044715 x^3))).
+"""
+import numpy as np
+
+K0 = np.sqrt(2.0 / np.pi)
+K1 = 0.044715
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + np.tanh(K0 * (x + K1 * x ** 3)))
+
+
+def gelu_grad(x):
+    t = np.tanh(K0 * (x + K1 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * K0 * (1.0 + 3.0 * K1 * x * x)
+
+
+def mode_lists(N, m):
+    ky = np.concatenate([np.arange(m), np.arange(N - m, N)])
+    kx = np.arange(m)
+    return ky, kx
+
+
+def spectral(h, R, N, m):
+    """h [C][S][N][N] real -> Re IDFT2(R . DFT2 h) with the retained modes.
+    R: [2m][m][Cout][Cin] complex."""
+    ky, kx = mode_lists(N, m)
+    F = np.fft.fft2(h, axes=(-2, -1))                    # [C][S][N][N]
+    B = F[:, :, ky][:, :, :, kx]                         # [C][S][2m][m]
+    O = np.einsum("yxoi,isyx->osyx", R, B)               # [Cout][S][2m][m]
+    full = np.zeros(O.shape[:2] + (N, N), complex)
+    w = np.where(kx == 0, 1.0, 2.0)
+    full[:, :, ky[:, None], kx[None, :]] = O * w
+    return np.real(np.fft.ifft2(full, axes=(-2, -1))), B
+
+
+def spectral_adjoint(g, R, B, N, m):
+    """adjoint of spectral() w.r.t. h and R for upstream gradient g [Cout][S][N][N]"""
+    ky, kx = mode_lists(N, m)
+    w = np.where(kx == 0, 1.0, 2.0)
+    # out = Re( sum_k w_k O_k e^{+i k.x} ) / P  ->  dO_k = w_k conj-free map:
+    # d out / d Ore_k = w_k cos / P, d out / d Oim_k = -w_k sin / P
+    G = np.fft.fft2(g, axes=(-2, -1)) / (N * N)          # sum_x g e^{-i k.x} / P
+    dO = np.conj(G[:, :, ky][:, :, :, kx]) * w           # dOre + i dOim packed as conj
+    dO = np.conj(dO)                                     # gradient w.r.t. (Ore, Oim) as complex dOre + i dOim
+    # O = R B  (complex, per mode): dB = R^H dO, dR = dO B^H (summed over samples)
+    dB = np.einsum("yxoi,osyx->isyx", np.conj(R), dO)
+    dR = np.einsum("osyx,isyx->yxoi", dO, np.conj(B))
+    # B = DFT2(h) restricted: dh = Re( sum_k dB_k^* ... ) -> real adjoint of the forward DFT
+    full = np.zeros(dB.shape[:2] + (N, N), complex)
+    full[:, :, ky[:, None], kx[None, :]] = dB
+    dh = np.real(np.fft.ifft2(full, axes=(-2, -1))) * (N * N)
+    return dh, dR
+
+
+class FNORef:
+    def __init__(self, params, N, m, blocks):
+        self.p = {k: np.asarray(v, np.float64) if not np.iscomplexobj(v) else np.asarray(v, np.complex128)
+                  for k, v in params.items()}
+        self.N, self.m, self.blocks = N, m, blocks
+
+    def forward(self, x):
+        """x [cin][S][N][N] -> y [cout][S][N][N]; keeps what backward needs"""
+        p, N, m = self.p, self.N, self.m
+        cache = {"x": x}
+        h = np.einsum("ci,isyx->csyx", p["Lw"], x) + p["Lb"][:, None, None, None]
+        for b in range(self.blocks):
+            cache[f"h{b}"] = h
+            s, B = spectral(h, p[f"R{b}"], N, m)
+            pre = np.einsum("oi,isyx->osyx", p[f"W{b}"], h) + s + p[f"B{b}"][:, None, None, None]
+            cache[f"pre{b}"], cache[f"B{b}"] = pre, B
+            h = gelu(pre) if b < self.blocks - 1 else pre
+        cache["hL"] = h
+        pre1 = np.einsum("oi,isyx->osyx", p["Q1"], h) + p["q1b"][:, None, None, None]
+        q1 = gelu(pre1)
+        cache["pre1"], cache["q1"] = pre1, q1
+        y = np.einsum("oi,isyx->osyx", p["Q2"], q1) + p["q2b"][:, None, None, None]
+        self.cache = cache
+        return y
+
+    def loss_and_grads(self, x, t):
+        p, N, m = self.p, self.N, self.m
+        y = self.forward(x)
+        c = self.cache
+        n = y.size
+        loss = np.mean((y - t) ** 2)
+        dy = 2.0 * (y - t) / n
+        g = {}
+        g["Q2"] = np.einsum("osyx,isyx->oi", dy, c["q1"])
+        g["q2b"] = dy.sum(axis=(1, 2, 3))
+        dq1 = np.einsum("oi,osyx->isyx", p["Q2"], dy)
+        dpre1 = dq1 * gelu_grad(c["pre1"])
+        g["Q1"] = np.einsum("osyx,isyx->oi", dpre1, c["hL"])
+        g["q1b"] = dpre1.sum(axis=(1, 2, 3))
+        dh = np.einsum("oi,osyx->isyx", p["Q1"], dpre1)
+        for b in reversed(range(self.blocks)):
+            pre = c[f"pre{b}"]
+            dpre = dh * gelu_grad(pre) if b < self.blocks - 1 else dh
+            h = c[f"h{b}"]
+            g[f"B{b}"] = dpre.sum(axis=(1, 2, 3))
+            g[f"W{b}"] = np.einsum("osyx,isyx->oi", dpre, h)
+            dhs, dR = spectral_adjoint(dpre, p[f"R{b}"], c[f"B{b}"], N, m)
+            g[f"R{b}"] = dR
+            dh = np.einsum("oi,osyx->isyx", p[f"W{b}"], dpre) + dhs
+        g["Lw"] = np.einsum("csyx,isyx->ci", dh, c["x"])
+        g["Lb"] = dh.sum(axis=(1, 2, 3))
+        return loss, g
