@@ -75,6 +75,7 @@ _SIGS = {
                                               C.POINTER(C.c_double)]),
     "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atsim_sim_advance": (C.c_int, [C.c_void_p]),
     "atsim_gemm_bf16": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_long, C.c_float,
                                   C.c_void_p, C.c_long, C.c_long, C.c_long,
                                   C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_float,
@@ -85,6 +86,11 @@ _SIGS = {
     "atsim_nn_bias_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_int]),
     "atsim_nn_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.POINTER(C.c_double)]),
     "atsim_nn_reduce_batches": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int]),
+    "atsim_nn_make_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                      C.c_long, C.c_long, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_double,
+                                      C.c_double, C.c_double]),
+    "atsim_nn_unpack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_double,
+                                  C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "atsim_nn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_long]),
 }

@@ -653,11 +653,11 @@ static void tap_weights(double f, int bicubic, double w[4]) {
     }
 }
 
-int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
-    if (!s || !d_pha || !d_amp) return ATSIM_EINVAL;
+// one time step of the layer positions (AT.c:2706-2755: cumulative FP64,
+// wrapped into [0, M]) and the upload of their integer bases and tap weights
+static int advance_positions(atsim_sim* s) {
     atsim_ctx* c = s->ctx;
-    const long L = s->L, D = s->D, W = s->W, M = s->M;
-    // positions: AT.c:2706-2755 (cumulative FP64, wrapped into [0, M])
+    const long L = s->L, D = s->D, M = s->M;
     LayerSample* slot = s->ls_host + (size_t)s->ls_slot * D * L;
     HIPCHK(c, hipEventSynchronize(s->ls_ev[s->ls_slot]));
     for (long d = 0; d < D; d++)
@@ -681,6 +681,19 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     HIPCHK(c, hipMemcpyAsync(s->ls_dev, slot, sizeof(LayerSample) * D * L, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipEventRecord(s->ls_ev[s->ls_slot], c->stream));
     s->ls_slot = (s->ls_slot + 1) % atsim_sim::RING;
+    return ATSIM_OK;
+}
+
+int atsim_sim_advance(atsim_sim* s) {
+    if (!s) return ATSIM_EINVAL;
+    return advance_positions(s);
+}
+
+int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
+    if (!s || !d_pha || !d_amp) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long L = s->L, D = s->D, W = s->W, M = s->M;
+    if (int r = advance_positions(s)) return r;
 
     ScreenArgs sa{};
     sa.masters = s->masters;
@@ -868,6 +881,22 @@ int atsim_nn_mse(atsim_ctx* c, const float* pred, const float* target, float* gr
 int atsim_nn_reduce_batches(atsim_ctx* c, const float* part, float* out, long nb, long n, int accumulate) {
     if (!c || !part || !out || nb <= 0 || n <= 0) return ATSIM_EINVAL;
     HIPCHK(c, launch_reduce_batches(part, out, nb, n, accumulate, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_make_pairs(atsim_ctx* c, const float* screens, const double* d_scoeff, const float* pha,
+                        const float* amp, long D, long L, long P, float* x, float* t, long Stot, long s0, double xs,
+                        double cs, double ps) {
+    if (!c || !screens || !d_scoeff || !x || D <= 0 || L <= 0 || P <= 0 || s0 < 0 || s0 + D > Stot) return ATSIM_EINVAL;
+    if (!(xs > 0) || !(cs > 0) || !(ps > 0)) return fail(c, ATSIM_EINVAL, "make_pairs: scales must be > 0");
+    HIPCHK(c, launch_nn_pairs(screens, d_scoeff, pha, amp, D, L, P, Stot, s0, x, t, xs, cs, ps, c->stream));
+    return ATSIM_OK;
+}
+
+int atsim_nn_unpack(atsim_ctx* c, const float* y, const float* x, long L, long S, long P, double xs, double cs,
+                    double ps, float* amp, float* pha) {
+    if (!c || !y || !x || !amp || !pha || L <= 0 || S <= 0 || P <= 0) return ATSIM_EINVAL;
+    HIPCHK(c, launch_nn_unpack(y, x, L, S, P, xs, cs, ps, amp, pha, c->stream));
     return ATSIM_OK;
 }
 

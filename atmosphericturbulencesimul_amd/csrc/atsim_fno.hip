@@ -206,6 +206,44 @@ __global__ void k_reduce_batches(const float* __restrict__ part, float* __restri
     }
 }
 
+// Training / serving pairs of the surrogate from one step of the simulator.
+// screens [D][L][P] (sampled layer screens), s[L] phase multipliers of the
+// wavelength; for sample s0 + d:
+//   x[l][s][p] = s_l screen_l / xs                    (network input)
+//   t[0][s][p] = log(amp) / cs, t[1][s][p] = (pha - sum_l s_l screen_l) / ps
+// (targets only when pha and amp are given), channel-major over Stot samples.
+__global__ void k_nn_pairs(const float* __restrict__ screens, const double* __restrict__ sc,
+                           const float* __restrict__ pha, const float* __restrict__ amp, int L, long P, long Stot,
+                           long s0, float* __restrict__ x, float* __restrict__ t, double xs, double cs, double ps) {
+    const int d = blockIdx.y;
+    const long s = s0 + d;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < P; p += (long)gridDim.x * blockDim.x) {
+        double phi = 0.0;
+        for (int l = 0; l < L; l++) {
+            const double v = sc[l] * (double)screens[((long)d * L + l) * P + p];
+            phi += v;
+            x[((long)l * Stot + s) * P + p] = (float)(v / xs);
+        }
+        if (t && pha && amp) {
+            const float a = fmaxf(amp[(long)d * P + p], 1e-30f);
+            t[s * P + p] = (float)(log((double)a) / cs);
+            t[(Stot + s) * P + p] = (float)(((double)pha[(long)d * P + p] - phi) / ps);
+        }
+    }
+}
+
+// network output y [2][S][P] -> amp = exp(cs y0), pha = xs sum_l x_l + ps y1
+__global__ void k_nn_unpack(const float* __restrict__ y, const float* __restrict__ x, int L, long S, long P,
+                            double xs, double cs, double ps, float* __restrict__ amp, float* __restrict__ pha) {
+    const long n = S * P;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        double phi = 0.0;
+        for (int l = 0; l < L; l++) phi += (double)x[(long)l * n + e];
+        amp[e] = (float)exp(cs * (double)y[e]);
+        pha[e] = (float)(xs * phi + ps * (double)y[n + e]);
+    }
+}
+
 static unsigned grid_for(long n) {
     long g = (n + 255) / 256;
     return (unsigned)(g < 8192 ? (g > 0 ? g : 1) : 8192);
@@ -231,6 +269,19 @@ hipError_t launch_mse(const float* pred, const float* target, float* grad, long 
 }
 hipError_t launch_reduce_batches(const float* part, float* out, long nb, long n, int accumulate, hipStream_t st) {
     k_reduce_batches<<<grid_for(n), 256, 0, st>>>(part, out, nb, n, accumulate);
+    return hipGetLastError();
+}
+hipError_t launch_nn_pairs(const float* screens, const double* sc, const float* pha, const float* amp, long D, long L,
+                           long P, long Stot, long s0, float* x, float* t, double xs, double cs, double ps,
+                           hipStream_t st) {
+    long g = (P + 255) / 256;
+    dim3 grid((unsigned)(g < 2048 ? g : 2048), (unsigned)D);
+    k_nn_pairs<<<grid, 256, 0, st>>>(screens, sc, pha, amp, (int)L, P, Stot, s0, x, t, xs, cs, ps);
+    return hipGetLastError();
+}
+hipError_t launch_nn_unpack(const float* y, const float* x, long L, long S, long P, double xs, double cs, double ps,
+                            float* amp, float* pha, hipStream_t st) {
+    k_nn_unpack<<<grid_for(S * P), 256, 0, st>>>(y, x, (int)L, S, P, xs, cs, ps, amp, pha);
     return hipGetLastError();
 }
 hipError_t launch_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,

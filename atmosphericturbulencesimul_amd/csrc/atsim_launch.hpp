@@ -72,6 +72,11 @@ hipError_t launch_bias_grad(const float* dy, float* db, long S, long C, long P, 
 hipError_t launch_mse(const float* pred, const float* target, float* grad, long n, double* partial, int nblk,
                       hipStream_t st);
 hipError_t launch_reduce_batches(const float* part, float* out, long nb, long n, int accumulate, hipStream_t st);
+hipError_t launch_nn_pairs(const float* screens, const double* sc, const float* pha, const float* amp, long D, long L,
+                           long P, long Stot, long s0, float* x, float* t, double xs, double cs, double ps,
+                           hipStream_t st);
+hipError_t launch_nn_unpack(const float* y, const float* x, long L, long S, long P, double xs, double cs, double ps,
+                            float* amp, float* pha, hipStream_t st);
 hipError_t launch_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
                        float c1, float c2, hipStream_t st);
 

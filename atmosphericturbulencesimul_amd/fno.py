@@ -51,7 +51,15 @@ class Ops:
 
     def gemm(self, M, N, K, batch, A, a_off, sab, sam, sak, B, b_off, sbb, sbk, sbn, Cm, c_off, scb, scm, scn,
              alpha=1.0, beta=0.0):
-        """C = alpha A B + beta C on float32 tensors with element offsets and strides"""
+        """C = alpha A B + beta C on float32 tensors with element offsets and strides;
+        raises before the launch if any operand's last element lies outside its tensor"""
+        if M > 0 and N > 0 and batch > 0:
+            last = lambda off, sb, d: off + (batch - 1) * sb + sum((n - 1) * st for n, st in d)
+            if (min(a_off, b_off, c_off) < 0
+                    or last(a_off, sab, [(M, sam), (K, sak)]) >= A.numel()
+                    or last(b_off, sbb, [(K, sbk), (N, sbn)]) >= B.numel()
+                    or last(c_off, scb, [(M, scm), (N, scn)]) >= Cm.numel()):
+                raise ValueError(f"gemm operands out of bounds (M {M} N {N} K {K} batch {batch})")
         L = self.L
         pa = C.c_void_p(A.data_ptr() + 4 * a_off)
         pb = C.c_void_p(B.data_ptr() + 4 * b_off)
@@ -60,25 +68,38 @@ class Ops:
                                         beta, pc, scb, scm, scn))
 
     def add_bias_act(self, a, b, bias, pre, y, Cc, P, gelu):
+        n = a.numel()
+        if any(t is not None and t.numel() != n for t in (b, pre, y)) or (bias is not None and bias.numel() != Cc):
+            raise ValueError("add_bias_act: operand sizes differ")
         self.ctx._chk(self.L.atsim_nn_add_bias_act(self.ctx.h, _p(a), _p(b), _p(bias), _p(pre), _p(y), Cc, P,
                                                    a.numel(), int(gelu)))
 
     def gelu_bwd(self, pre, dy, dx):
+        if not pre.numel() == dy.numel() == dx.numel():
+            raise ValueError(f"gelu_bwd: sizes {pre.numel()}, {dy.numel()}, {dx.numel()} differ")
         self.ctx._chk(self.L.atsim_nn_gelu_bwd(self.ctx.h, _p(pre), _p(dy), _p(dx), pre.numel()))
 
     def bias_grad(self, dy, db, Cc, P):
+        if dy.numel() != Cc * P or db.numel() != Cc:
+            raise ValueError("bias_grad: operand sizes differ")
         self.ctx._chk(self.L.atsim_nn_bias_grad(self.ctx.h, _p(dy), _p(db), 1, Cc, P, 0))
 
     def mse(self, pred, target, grad, want_loss=True):
+        if target.numel() != pred.numel() or (grad is not None and grad.numel() != pred.numel()):
+            raise ValueError("mse: operand sizes differ")
         loss = C.c_double()
         self.ctx._chk(self.L.atsim_nn_mse(self.ctx.h, _p(pred), _p(target), _p(grad), pred.numel(),
                                           C.byref(loss) if want_loss else None))
-        return loss.value
+        return loss.value if want_loss else None
 
     def reduce(self, part, out, nb, n):
+        if part.numel() < nb * n or out.numel() < n:
+            raise ValueError("reduce: operands too small")
         self.ctx._chk(self.L.atsim_nn_reduce_batches(self.ctx.h, _p(part), _p(out), nb, n, 0))
 
     def adam(self, w, g, m, v, lr, b1, b2, eps, step):
+        if not w.numel() == g.numel() == m.numel() == v.numel():
+            raise ValueError("adam: operand sizes differ")
         self.ctx._chk(self.L.atsim_nn_adam(self.ctx.h, _p(w), _p(g), _p(m), _p(v), w.numel(), lr, b1, b2, eps,
                                            step))
 
@@ -105,12 +126,13 @@ class FNO:
     """The network on the GPU: parameters, forward, backward, Adam."""
 
     @on_ctx_stream
-    def __init__(self, ctx, N, cin, cout, width=32, modes=16, blocks=4, hidden=64, params=None, seed=0):
+    def __init__(self, ctx, N, cin, cout, width=32, modes=16, blocks=4, hidden=64, params=None, seed=0,
+                 device=None):
         torch = _torch()
         self.ctx, self.ops = ctx, Ops(ctx)
         self.N, self.cin, self.cout, self.width, self.m, self.blocks, self.hidden = N, cin, cout, width, modes, \
             blocks, hidden
-        self.dev = f"cuda:{ctx.device}"
+        self.dev = device or f"cuda:{ctx.device}"
         if params is None:
             params = init_params(cin, cout, width, modes, blocks, hidden, seed)
         self.param = {}
@@ -273,20 +295,21 @@ class FNO:
             h = hn
         cache["hL"] = h
         q1 = torch.empty((self.hidden, S, self.N, self.N), dtype=torch.float32, device=self.dev)
-        pre1 = torch.empty_like(q1) if keep else None
+        preq = torch.empty_like(q1) if keep else None
         self._pointwise(p["Q1"], h, q1, self.hidden, Cw, SP)
-        ops.add_bias_act(q1, None, p["q1b"], pre1, q1, self.hidden, SP, True)
+        ops.add_bias_act(q1, None, p["q1b"], preq, q1, self.hidden, SP, True)
         y = torch.empty((self.cout, S, self.N, self.N), dtype=torch.float32, device=self.dev)
         self._pointwise(p["Q2"], q1, y, self.cout, self.hidden, SP)
         ops.add_bias_act(y, None, p["q2b"], None, y, self.cout, SP, False)
         if keep:
-            cache["pre1"], cache["q1"] = pre1, q1
+            cache["preq"], cache["q1"] = preq, q1
             self.cache = cache
         return y
 
     @on_ctx_stream
-    def loss_and_grads(self, x, t):
-        """forward + backward; gradients land in self.grad; returns the MSE loss"""
+    def loss_and_grads(self, x, t, want_loss=True):
+        """forward + backward; gradients land in self.grad; returns the MSE loss
+        (None without want_loss, which spares a stream synchronisation)"""
         torch = _torch()
         y = self.forward(x, keep=True)
         c, p, g, ops = self.cache, self.param, self.grad, self.ops
@@ -295,12 +318,12 @@ class FNO:
         SP = S * P
         Cw, H = self.width, self.hidden
         dy = torch.empty_like(y)
-        loss = ops.mse(y, t, dy)
+        loss = ops.mse(y, t, dy, want_loss)
         self._weight_grad(dy, c["q1"], g["Q2"], self.cout, H, SP)
         ops.bias_grad(dy, g["q2b"], self.cout, SP)
         dq1 = torch.empty_like(c["q1"])
         self._pointwise_t(p["Q2"], dy, dq1, self.cout, H, SP)
-        ops.gelu_bwd(c["pre1"], dq1, dq1)
+        ops.gelu_bwd(c["preq"], dq1, dq1)
         self._weight_grad(dq1, c["hL"], g["Q1"], H, Cw, SP)
         ops.bias_grad(dq1, g["q1b"], H, SP)
         dh = torch.empty_like(c["hL"])

@@ -1,0 +1,339 @@
+"""Learned surrogate of the layer loop: training on simulator-generated pairs
+and judged serving (no counterpart in the reference).
+
+Workload (256^2 pupil, the 7-layer profile at 0/0.5/1/2/4/8/16 km with Cn2
+fractions 0.4/0.15/0.1/0.1/0.1/0.1/0.05, r0 = 0.12 m at 0.5 um, 4096^2
+masters, science wavelength 0.7 um):
+* input: the 7 sampled layer screens scaled to phase at 0.7 um;
+* output: pupil log-amplitude chi = log|E| and the diffraction phase
+  dphi = pha - sum_l phase_l (the wrapped difference between the propagated
+  phase and the geometric phase), so the field is exp(chi + i (Phi + dphi)).
+Pixels are 5 mm (1.28 m pupil) so that the Fresnel scales of the upper
+layers, sqrt(lambda z) = 7.5-10.6 cm at 8-16 km, lie inside the 16 retained
+modes; the network cannot represent structure finer than its modes.
+
+Training pairs come from a simulation with 16 directions on a 2 x 2 arcmin
+grid and 50 ms steps (decorrelated frames).  Serving runs a sequence of
+1 ms frames: every frame advances the layers, samples the screens and runs
+the network; every `judge_every`-th frame the plain Fresnel path computes
+the same frame, and the network's frames are replaced by plain ones for the
+next block when the judged field error exceeds `fallback_err`.
+
+    python -m atmosphericturbulencesimul_amd.surrogate [--pairs 50000] [--frames 10000]
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from .fno import FNO, on_ctx_stream
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def workload(dx=0.005):
+    L = 7
+    return dict(
+        N=256, M=4096, L=L, dx=dx, lam=0.7e-6, lam_ref=0.5e-6, r0=0.12, L0=20.0, l0=0.01, bin_m=1000.0,
+        alts=np.array([0.0, 500.0, 1000.0, 2000.0, 4000.0, 8000.0, 16000.0]),
+        cn2=np.array([0.4, 0.15, 0.1, 0.1, 0.1, 0.1, 0.05]),
+        speeds=np.linspace(5.0, 40.0, L),
+        dirs=np.random.default_rng(0xA75 + 5000).uniform(0.0, 2 * math.pi, L))
+
+
+def make_sim(ctx, wl, seed, dt, directions):
+    from .api import Simulation, superlayers
+    L, M, dx = wl["L"], wl["M"], wl["dx"]
+    cn2 = wl["cn2"] / wl["cn2"].sum()
+    sa, sli = superlayers(wl["alts"], cn2, wl["bin_m"])
+    D = len(directions)
+    x0 = np.array([[0.05 * M + tx * a / dx for a in wl["alts"]] for tx, _ in directions])
+    y0 = np.array([[0.05 * M + ty * a / dx for a in wl["alts"]] for _, ty in directions])
+    vx = wl["speeds"] * np.cos(wl["dirs"]) / dx
+    vy = wl["speeds"] * np.sin(wl["dirs"]) / dx
+    scoeff = np.array([[wl["lam_ref"] / wl["lam"]] * L])
+    sim = Simulation(ctx, wl["N"], wl["N"], M, sa, sli, [wl["lam"]], scoeff, [0], x0, y0, vx, vy, dx, dt, 0.0,
+                     True)
+    sim.synth_masters(seed, wl["L0"] / dx, wl["l0"] / dx)
+    sim.normalize_masters(wl["r0"] / dx, cn2)
+    return sim, scoeff[0]
+
+
+def grid_directions(n=4, half_arcmin=1.0):
+    a = np.linspace(-half_arcmin, half_arcmin, n) * math.pi / (180 * 60)
+    return [(float(x), float(y)) for y in a for x in a]
+
+
+class PairSource:
+    """Pairs (x, t) from a simulation, channel-major [C][S][P] on the GPU."""
+
+    def __init__(self, ctx, sim, scoeff, wl):
+        torch = _torch()
+        self.ctx, self.sim, self.wl = ctx, sim, wl
+        self.L, self.N = wl["L"], wl["N"]
+        self.P = self.N * self.N
+        self.D = sim.D
+        self.scr = torch.empty((self.D, self.L, self.P), dtype=torch.float32, device=f"cuda:{ctx.device}")
+        self.sc = torch.tensor(np.asarray(scoeff, np.float64), device=f"cuda:{ctx.device}")
+        torch.cuda.synchronize()
+        self.scales = (1.0, 1.0, 1.0)
+
+    def fill(self, X, T, Stot, frames, plain=True, s_start=0):
+        """frames simulator steps into samples s_start.. of X [L][Stot][P] (and
+        T [2][Stot][P]); X and T must be contiguous.  plain=False only
+        advances the layers (no Fresnel frame, no targets)"""
+        assert X.is_contiguous() and (T is None or T.is_contiguous())
+        L_ = self.ctx.L
+        xs, cs, ps = self.scales
+        for f in range(frames):
+            if plain:
+                pha, amp = self.sim.frame(sync=False)
+            else:
+                self.ctx._chk(L_.atsim_sim_advance(self.sim.h))
+                pha = amp = None
+            self.ctx._chk(L_.atsim_sim_layer_screens(self.sim.h, self.scr.data_ptr()))
+            p = lambda t: t.data_ptr() if t is not None else None
+            self.ctx._chk(L_.atsim_nn_make_pairs(self.ctx.h, self.scr.data_ptr(), self.sc.data_ptr(), p(pha),
+                                                 p(amp), self.D, self.L, self.P, X.data_ptr(),
+                                                 T.data_ptr() if T is not None else None, Stot,
+                                                 s_start + f * self.D,
+                                                 xs, cs, ps))
+
+
+def rms(ctx, t):
+    from .fno import Ops
+    torch = _torch()
+    z = torch.zeros_like(t)
+    with torch.cuda.stream(ctx.stream):
+        return math.sqrt(Ops(ctx).mse(t, z, None))
+
+
+def train(ctx, net, src, pairs, frames_per_chunk=32, batch=8, lr=1e-3, epochs_per_chunk=1, log=print, seed=0):
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    P, L = src.P, src.L
+    Sc = frames_per_chunk * src.D
+    nchunks = max(1, pairs // Sc)
+    steps_total = nchunks * epochs_per_chunk * (Sc // batch)
+    rng = np.random.default_rng(seed)
+    losses = []
+    step = 0
+    t0 = time.perf_counter()
+    with torch.cuda.stream(ctx.stream):
+        X = torch.empty((L, Sc, P), dtype=torch.float32, device=dev)
+        T = torch.empty((2, Sc, P), dtype=torch.float32, device=dev)
+        for ch in range(nchunks):
+            src.fill(X, T, Sc, frames_per_chunk)
+            for _ in range(epochs_per_chunk):
+                perm = rng.permutation(Sc)
+                for b0 in range(0, Sc - batch + 1, batch):
+                    idx = torch.tensor(perm[b0:b0 + batch], device=dev)
+                    xb = X.index_select(1, idx).contiguous()
+                    tb = T.index_select(1, idx).contiguous()
+                    loss = net.loss_and_grads(xb, tb, want_loss=(step % 16 == 0))
+                    frac = step / max(1, steps_total - 1)
+                    net.adam_step(lr=lr * (0.02 + 0.98 * 0.5 * (1 + math.cos(math.pi * frac))))
+                    if loss is not None:
+                        losses.append(loss)
+                    step += 1
+            log(f"chunk {ch + 1}/{nchunks}: pairs {(ch + 1) * Sc}, loss {np.mean(losses[-8:]):.4f}, "
+                f"{time.perf_counter() - t0:.1f} s")
+    ctx.sync()
+    return losses, time.perf_counter() - t0
+
+
+def scint_index(amp):
+    I = amp.astype(np.float64) ** 2
+    return float(I.var() / I.mean() ** 2)
+
+
+def strfn_x(pha, lags):
+    p = pha.astype(np.float64)
+    return np.array([np.mean((p[:, r:] - p[:, :-r]) ** 2) for r in lags])
+
+
+def serve(ctx, net, wl, scales, frames=10000, judge_every=50, fallback_err=0.5, seed=0xA75 + 5100, log=print):
+    """judged serving of a 1 ms sequence; returns the report dictionary"""
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    sim, scoeff = make_sim(ctx, wl, seed, 0.001, [(0.0, 0.0)])
+    src = PairSource(ctx, sim, scoeff, wl)
+    src.scales = scales
+    xs, cs, ps = scales
+    P, L, N = src.P, src.L, src.N
+    B = judge_every
+    lags = [1, 2, 4, 8, 16, 32]
+    rec = dict(si_plain=[], si_fno=[], sf_plain=[], sf_fno=[], err_fno=[], err_geo=[])
+    fallback_blocks = 0
+    fallback = False
+    t_fno = 0.0
+    n_fno = 0
+    with torch.cuda.stream(ctx.stream):
+        X = torch.empty((L, B, P), dtype=torch.float32, device=dev)
+        amp_o = torch.empty((B, P), dtype=torch.float32, device=dev)
+        pha_o = torch.empty((B, P), dtype=torch.float32, device=dev)
+        nblocks = frames // B
+        for k in range(nblocks):
+            if fallback:
+                # plain frames for this block; the first is judged again
+                for f in range(B):
+                    pha_p, amp_p = sim.frame(sync=False)
+                fallback_blocks += 1
+                fallback = False
+                continue
+            ctx.sync()
+            t0 = time.perf_counter()
+            # frame 0 of the block is the judged one: the plain path computes it
+            src.fill(X, None, B, 1, plain=True, s_start=0)
+            pha_p = sim.pha.clone()
+            amp_p = sim.amp.clone()
+            src.fill(X, None, B, B - 1, plain=False, s_start=1)
+            y = net.forward(X, keep=False)
+            ctx._chk(ctx.L.atsim_nn_unpack(ctx.h, y.data_ptr(), X.data_ptr(), L, B, P, xs, cs, ps,
+                                            amp_o.data_ptr(), pha_o.data_ptr()))
+            ctx.sync()
+            t_fno += time.perf_counter() - t0
+            n_fno += B
+            # judge (host side)
+            ap = amp_p.cpu().numpy().reshape(N, N)
+            pp = pha_p.cpu().numpy().reshape(N, N)
+            af = amp_o[0].cpu().numpy().reshape(N, N)
+            pf = pha_o[0].cpu().numpy().reshape(N, N)
+            phi = xs * X[:, 0].sum(0).cpu().numpy().reshape(N, N)
+            Ep = ap * np.exp(1j * pp)
+            Ef = af * np.exp(1j * pf)
+            Eg = np.exp(1j * phi)
+            nrm = np.linalg.norm(Ep)
+            e_f = float(np.linalg.norm(Ef - Ep) / nrm)
+            e_g = float(np.linalg.norm(Eg - Ep) / nrm)
+            rec["si_plain"].append(scint_index(ap))
+            rec["si_fno"].append(scint_index(af))
+            rec["sf_plain"].append(strfn_x(pp, lags))
+            rec["sf_fno"].append(strfn_x(pf, lags))
+            rec["err_fno"].append(e_f)
+            rec["err_geo"].append(e_g)
+            if e_f > fallback_err:
+                fallback = True
+            if (k + 1) % 20 == 0:
+                log(f"block {k + 1}/{nblocks}: field err {np.mean(rec['err_fno'][-20:]):.3f} "
+                    f"(geometric {np.mean(rec['err_geo'][-20:]):.3f})")
+    sim.close()
+    si_p, si_f = np.mean(rec["si_plain"]), np.mean(rec["si_fno"])
+    sf_p, sf_f = np.mean(rec["sf_plain"], 0), np.mean(rec["sf_fno"], 0)
+    return dict(
+        frames=frames, judge_every=B, judged=len(rec["err_fno"]), fallback_blocks=fallback_blocks,
+        scint_index_plain=si_p, scint_index_fno=si_f, scint_index_rel_err=abs(si_f - si_p) / si_p,
+        strfn_lags_px=lags, strfn_plain=sf_p.tolist(), strfn_fno=sf_f.tolist(),
+        strfn_rel_err=(np.abs(sf_f - sf_p) / sf_p).tolist(),
+        field_rel_err_fno=float(np.mean(rec["err_fno"])), field_rel_err_geometric=float(np.mean(rec["err_geo"])),
+        fno_frames_per_s=n_fno / t_fno if t_fno > 0 else None)
+
+
+def plain_rate(ctx, wl, frames=500, seed=0xA75 + 5200):
+    """frames/s of the plain path on the same 256^2, 7-layer, one-wavelength frame"""
+    sim, _ = make_sim(ctx, wl, seed, 0.001, [(0.0, 0.0)])
+    for _ in range(10):
+        sim.frame(sync=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.frame(sync=False)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    sim.close()
+    return frames / el
+
+
+def fno_rate(ctx, net, wl, scales, frames=500, batch=50, seed=0xA75 + 5300):
+    """frames/s of the network path (advance, screens, pairs, forward, unpack) without judging"""
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    sim, scoeff = make_sim(ctx, wl, seed, 0.001, [(0.0, 0.0)])
+    src = PairSource(ctx, sim, scoeff, wl)
+    src.scales = scales
+    xs, cs, ps = scales
+    P, L = src.P, src.L
+    with torch.cuda.stream(ctx.stream):
+        X = torch.empty((L, batch, P), dtype=torch.float32, device=dev)
+        amp_o = torch.empty((batch, P), dtype=torch.float32, device=dev)
+        pha_o = torch.empty((batch, P), dtype=torch.float32, device=dev)
+
+        def block():
+            src.fill(X, None, batch, batch, plain=False)
+            y = net.forward(X, keep=False)
+            ctx._chk(ctx.L.atsim_nn_unpack(ctx.h, y.data_ptr(), X.data_ptr(), L, batch, P, xs, cs, ps,
+                                            amp_o.data_ptr(), pha_o.data_ptr()))
+        block()
+        ctx.sync()
+        nb = max(1, frames // batch)
+        t0 = time.perf_counter()
+        for _ in range(nb):
+            block()
+        ctx.sync()
+        el = time.perf_counter() - t0
+    sim.close()
+    return nb * batch / el
+
+
+def calibrate(ctx, src, frames=4):
+    """normalisation scales (RMS of input phase, log-amplitude, diffraction phase)"""
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    Sc = frames * src.D
+    with torch.cuda.stream(ctx.stream):
+        X = torch.empty((src.L, Sc, src.P), dtype=torch.float32, device=dev)
+        T = torch.empty((2, Sc, src.P), dtype=torch.float32, device=dev)
+        src.scales = (1.0, 1.0, 1.0)
+        src.fill(X, T, Sc, frames)
+        xs = rms(ctx, X)
+        cs = rms(ctx, T[0])
+        ps = rms(ctx, T[1])
+    return (xs, cs, ps)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=50000)
+    ap.add_argument("--frames", type=int, default=10000)
+    ap.add_argument("--judge-every", type=int, default=50)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--epochs-per-chunk", type=int, default=1)
+    ap.add_argument("--width", type=int, default=32)
+    ap.add_argument("--modes", type=int, default=16)
+    ap.add_argument("--dx", type=float, default=0.005)
+    args = ap.parse_args(argv)
+    from .api import Context
+    ctx = Context(0)
+    wl = workload(args.dx)
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    sim, scoeff = make_sim(ctx, wl, 0xA75 + 5000, 0.05, grid_directions())
+    src = PairSource(ctx, sim, scoeff, wl)
+    scales = calibrate(ctx, src)
+    src.scales = scales
+    log(f"scales: input phase {scales[0]:.3f} rad, log-amplitude {scales[1]:.4f}, diffraction phase {scales[2]:.4f}")
+    net = FNO(ctx, wl["N"], wl["L"], 2, width=args.width, modes=args.modes, blocks=4, hidden=64, seed=1)
+    losses, t_train = train(ctx, net, src, args.pairs, batch=args.batch, lr=args.lr,
+                            epochs_per_chunk=args.epochs_per_chunk, log=log)
+    sim.close()
+    rep = serve(ctx, net, wl, scales, frames=args.frames, judge_every=args.judge_every, log=log)
+    rep["plain_frames_per_s"] = plain_rate(ctx, wl)
+    rep["fno_frames_per_s_unjudged"] = fno_rate(ctx, net, wl, scales)
+    rep["speedup_vs_plain"] = rep["fno_frames_per_s_unjudged"] / rep["plain_frames_per_s"]
+    rep.update(train_pairs=args.pairs, train_s=t_train,
+               loss_first=float(np.mean(losses[:4])), loss_last=float(np.mean(losses[-8:])),
+               network=dict(blocks=4, width=args.width, modes=[2 * args.modes, args.modes], hidden=64,
+                            params=int(sum(v.numel() for v in net.param.values()))),
+               workload=dict(N=wl["N"], layers=wl["L"], lam_um=wl["lam"] * 1e6, dx_m=wl["dx"], M=wl["M"]))
+    print(json.dumps(rep))
+
+
+if __name__ == "__main__":
+    main()

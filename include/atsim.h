@@ -181,6 +181,10 @@ long atsim_sim_gaps(atsim_sim *sim);
 int atsim_sim_profile(atsim_sim *sim, int enable);
 int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
 
+/* Advance every layer by one time step without computing a frame (the
+ * positions a following atsim_sim_layer_screens samples).  No counterpart in
+ * the reference: the surrogate's frames skip the layer loop. */
+int atsim_sim_advance(atsim_sim *sim);
 /* Per-layer sampled screens at the positions of the last frame (the FP64
  * sampler of the frame loop, rounded to float): d_out [D][L][N][N], radians
  * at the reference wavelength before Scoeff.  No counterpart in the
@@ -212,6 +216,18 @@ int atsim_nn_bias_grad(atsim_ctx *ctx, const float *dy, float *db, long S, long 
 /* mean squared error; grad (may be NULL) = 2 (pred - target) / n; loss (may
  * be NULL; synchronises the stream) = mean of (pred - target)^2 */
 int atsim_nn_mse(atsim_ctx *ctx, const float *pred, const float *target, float *grad, long n, double *loss);
+/* Pairs of the surrogate from one simulator step: screens [D][L][P] (from
+ * atsim_sim_layer_screens), d_scoeff [L] (device, FP64) the wavelength's
+ * phase multipliers; writes for samples s0 .. s0+D-1 of Stot, channel-major:
+ * x[l][s][p] = scoeff_l screen_l / xs and, when pha and amp (each [D][P]) are
+ * given, t[0][s][p] = log(amp) / cs, t[1][s][p] = (pha - sum_l scoeff_l
+ * screen_l) / ps. */
+int atsim_nn_make_pairs(atsim_ctx *ctx, const float *screens, const double *d_scoeff, const float *pha,
+                        const float *amp, long D, long L, long P, float *x, float *t, long Stot, long s0,
+                        double xs, double cs, double ps);
+/* network output y [2][S][P] -> amp = exp(cs y0), pha = xs sum_l x_l + ps y1 */
+int atsim_nn_unpack(atsim_ctx *ctx, const float *y, const float *x, long L, long S, long P, double xs, double cs,
+                    double ps, float *amp, float *pha);
 /* out[i] = sum_{b < nb} part[b n + i] (added to out if accumulate) */
 int atsim_nn_reduce_batches(atsim_ctx *ctx, const float *part, float *out, long nb, long n, int accumulate);
 /* one Adam step (bias-corrected, step >= 1) on n parameters */
