@@ -88,9 +88,9 @@ class FNORef:
             cache[f"pre{b}"], cache[f"B{b}"] = pre, B
             h = gelu(pre) if b < self.blocks - 1 else pre
         cache["hL"] = h
-        pre1 = np.einsum("oi,isyx->osyx", p["Q1"], h) + p["q1b"][:, None, None, None]
-        q1 = gelu(pre1)
-        cache["pre1"], cache["q1"] = pre1, q1
+        preq = np.einsum("oi,isyx->osyx", p["Q1"], h) + p["q1b"][:, None, None, None]
+        q1 = gelu(preq)
+        cache["preq"], cache["q1"] = preq, q1
         y = np.einsum("oi,isyx->osyx", p["Q2"], q1) + p["q2b"][:, None, None, None]
         self.cache = cache
         return y
@@ -106,7 +106,7 @@ class FNORef:
         g["Q2"] = np.einsum("osyx,isyx->oi", dy, c["q1"])
         g["q2b"] = dy.sum(axis=(1, 2, 3))
         dq1 = np.einsum("oi,osyx->isyx", p["Q2"], dy)
-        dpre1 = dq1 * gelu_grad(c["pre1"])
+        dpre1 = dq1 * gelu_grad(c["preq"])
         g["Q1"] = np.einsum("osyx,isyx->oi", dpre1, c["hL"])
         g["q1b"] = dpre1.sum(axis=(1, 2, 3))
         dh = np.einsum("oi,osyx->isyx", p["Q1"], dpre1)
