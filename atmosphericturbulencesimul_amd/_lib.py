@@ -80,6 +80,10 @@ _SIGS = {
                                   C.c_void_p, C.c_long, C.c_long, C.c_long,
                                   C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_float,
                                   C.c_void_p, C.c_long, C.c_long, C.c_long]),
+    "atsim_gemm_bf16_ex": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_long, C.c_float,
+                                     C.c_void_p, C.c_long, C.c_long, C.c_long,
+                                     C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_float,
+                                     C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_void_p, C.c_int]),
     "atsim_nn_add_bias_act": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_long, C.c_long, C.c_long, C.c_int]),
     "atsim_nn_gelu_bwd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]),
@@ -88,7 +92,7 @@ _SIGS = {
     "atsim_nn_reduce_batches": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int]),
     "atsim_nn_make_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
                                       C.c_long, C.c_long, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_double,
-                                      C.c_double, C.c_double]),
+                                      C.c_double, C.c_double, C.c_double]),
     "atsim_nn_unpack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_double,
                                   C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "atsim_nn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float,

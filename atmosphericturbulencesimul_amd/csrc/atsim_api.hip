@@ -27,6 +27,8 @@ struct atsim_ctx {
     int row_mw = 0;  // wavelengths per row-pass workgroup
     int col_c = 0;   // columns per column-pass workgroup
     double* nn_part = nullptr;  // 256 partial sums of the network's loss reduction
+    float* nn_bias = nullptr;   // per-chunk partial sums of bias gradients
+    size_t nn_bias_cap = 0;
 };
 
 static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
@@ -101,6 +103,7 @@ void atsim_ctx_destroy(atsim_ctx* c) {
     hipStreamSynchronize(c->stream);
     for (auto& kv : c->twiddles) hipFree(kv.second);
     if (c->nn_part) hipFree(c->nn_part);
+    if (c->nn_bias) hipFree(c->nn_bias);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -837,8 +840,16 @@ int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {
 int atsim_gemm_bf16(atsim_ctx* c, long M, long N, long K, long batch, float alpha, const float* A, long sab,
                     long sam, long sak, const float* B, long sbb, long sbk, long sbn, float beta, float* C, long scb,
                     long scm, long scn) {
+    return atsim_gemm_bf16_ex(c, M, N, K, batch, alpha, A, sab, sam, sak, B, sbb, sbk, sbn, beta, C, scb, scm, scn,
+                              nullptr, nullptr, 0);
+}
+
+int atsim_gemm_bf16_ex(atsim_ctx* c, long M, long N, long K, long batch, float alpha, const float* A, long sab,
+                       long sam, long sak, const float* B, long sbb, long sbk, long sbn, float beta, float* C,
+                       long scb, long scm, long scn, const float* bias, float* pre, int gelu) {
     if (!c || !A || !B || !C || M < 0 || N < 0 || K < 0 || batch < 0) return ATSIM_EINVAL;
-    GemmArgs g{M, N, K, batch, alpha, beta, A, sab, sam, sak, B, sbb, sbk, sbn, C, scb, scm, scn};
+    GemmArgs g{M, N, K, batch, alpha, beta, A, sab, sam, sak, B, sbb, sbk, sbn, C, scb, scm, scn, bias, pre,
+               gelu ? 1 : 0, 0, 0};
     hipError_t e = launch_gemm_bf16(g, c->stream);
     if (e == hipErrorInvalidValue) return fail(c, ATSIM_EUNSUPPORTED, "gemm: grid too large (M %ld, batch %ld)", M, batch);
     HIPCHK(c, e);
@@ -860,7 +871,17 @@ int atsim_nn_gelu_bwd(atsim_ctx* c, const float* pre, const float* dy, float* dx
 
 int atsim_nn_bias_grad(atsim_ctx* c, const float* dy, float* db, long S, long C, long P, int accumulate) {
     if (!c || !dy || !db || S <= 0 || C <= 0 || P <= 0) return ATSIM_EINVAL;
-    HIPCHK(c, launch_bias_grad(dy, db, S, C, P, accumulate, c->stream));
+    const int nchunk = 64;
+    const size_t need = sizeof(float) * nchunk * C;
+    if (c->nn_bias_cap < need) {
+        if (c->nn_bias) hipFree(c->nn_bias);
+        c->nn_bias = nullptr;
+        c->nn_bias_cap = 0;
+        HIPCHK(c, hipMalloc(&c->nn_bias, need));
+        c->nn_bias_cap = need;
+    }
+    HIPCHK(c, launch_bias_grad(dy, c->nn_bias, S, C, P, nchunk, c->stream));
+    HIPCHK(c, launch_reduce_batches(c->nn_bias, db, nchunk, C, accumulate, c->stream));
     return ATSIM_OK;
 }
 
@@ -885,11 +906,11 @@ int atsim_nn_reduce_batches(atsim_ctx* c, const float* part, float* out, long nb
 }
 
 int atsim_nn_make_pairs(atsim_ctx* c, const float* screens, const double* d_scoeff, const float* pha,
-                        const float* amp, long D, long L, long P, float* x, float* t, long Stot, long s0, double xs,
-                        double cs, double ps) {
-    if (!c || !screens || !d_scoeff || !x || D <= 0 || L <= 0 || P <= 0 || s0 < 0 || s0 + D > Stot) return ATSIM_EINVAL;
-    if (!(xs > 0) || !(cs > 0) || !(ps > 0)) return fail(c, ATSIM_EINVAL, "make_pairs: scales must be > 0");
-    HIPCHK(c, launch_nn_pairs(screens, d_scoeff, pha, amp, D, L, P, Stot, s0, x, t, xs, cs, ps, c->stream));
+                        const float* amp, long D, long L, long N, float* x, float* t, long Stot, long s0, double xs,
+                        double xl, double cs, double ps) {
+    if (!c || !screens || !d_scoeff || !x || D <= 0 || L <= 0 || N <= 1 || s0 < 0 || s0 + D > Stot) return ATSIM_EINVAL;
+    if (!(xs > 0) || !(xl > 0) || !(cs > 0) || !(ps > 0)) return fail(c, ATSIM_EINVAL, "make_pairs: scales must be > 0");
+    HIPCHK(c, launch_nn_pairs(screens, d_scoeff, pha, amp, D, L, N, Stot, s0, x, t, xs, xl, cs, ps, c->stream));
     return ATSIM_OK;
 }
 

@@ -63,17 +63,21 @@ struct GemmArgs {
     long sbb, sbk, sbn;
     float* C;
     long scb, scm, scn;
+    const float* bias;  // per-row bias (or null)
+    float* pre;         // pre-activation output with C's strides (or null)
+    int act;            // 1: GELU
+    int avec, bvec;     // staging modes, set by launch_gemm_bf16
 };
 hipError_t launch_gemm_bf16(const GemmArgs& g, hipStream_t st);
 hipError_t launch_add_bias_act(const float* a, const float* b, const float* bias, float* pre, float* y, long C,
                                long P, long n, int gelu, hipStream_t st);
 hipError_t launch_gelu_bwd(const float* pre, const float* dy, float* dx, long n, hipStream_t st);
-hipError_t launch_bias_grad(const float* dy, float* db, long S, long C, long P, int accumulate, hipStream_t st);
+hipError_t launch_bias_grad(const float* dy, float* partial, long S, long C, long P, int nchunk, hipStream_t st);
 hipError_t launch_mse(const float* pred, const float* target, float* grad, long n, double* partial, int nblk,
                       hipStream_t st);
 hipError_t launch_reduce_batches(const float* part, float* out, long nb, long n, int accumulate, hipStream_t st);
 hipError_t launch_nn_pairs(const float* screens, const double* sc, const float* pha, const float* amp, long D, long L,
-                           long P, long Stot, long s0, float* x, float* t, double xs, double cs, double ps,
+                           long N, long Stot, long s0, float* x, float* t, double xs, double xl, double cs, double ps,
                            hipStream_t st);
 hipError_t launch_nn_unpack(const float* y, const float* x, long L, long S, long P, double xs, double cs, double ps,
                             float* amp, float* pha, hipStream_t st);

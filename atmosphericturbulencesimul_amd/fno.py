@@ -50,22 +50,31 @@ class Ops:
         self.L = ctx.L
 
     def gemm(self, M, N, K, batch, A, a_off, sab, sam, sak, B, b_off, sbb, sbk, sbn, Cm, c_off, scb, scm, scn,
-             alpha=1.0, beta=0.0):
-        """C = alpha A B + beta C on float32 tensors with element offsets and strides;
-        raises before the launch if any operand's last element lies outside its tensor"""
+             alpha=1.0, beta=0.0, bias=None, pre=None, gelu=False):
+        """C = act(alpha A B + beta C + bias[m]) on float32 tensors with element
+        offsets and strides (pre, if given, receives the value before the
+        activation); raises before the launch if any operand's last element
+        lies outside its tensor"""
         if M > 0 and N > 0 and batch > 0:
             last = lambda off, sb, d: off + (batch - 1) * sb + sum((n - 1) * st for n, st in d)
             if (min(a_off, b_off, c_off) < 0
                     or last(a_off, sab, [(M, sam), (K, sak)]) >= A.numel()
                     or last(b_off, sbb, [(K, sbk), (N, sbn)]) >= B.numel()
-                    or last(c_off, scb, [(M, scm), (N, scn)]) >= Cm.numel()):
+                    or last(c_off, scb, [(M, scm), (N, scn)]) >= Cm.numel()
+                    or (bias is not None and bias.numel() < M)
+                    or (pre is not None and pre.numel() != Cm.numel())):
                 raise ValueError(f"gemm operands out of bounds (M {M} N {N} K {K} batch {batch})")
         L = self.L
         pa = C.c_void_p(A.data_ptr() + 4 * a_off)
         pb = C.c_void_p(B.data_ptr() + 4 * b_off)
         pc = C.c_void_p(Cm.data_ptr() + 4 * c_off)
-        self.ctx._chk(L.atsim_gemm_bf16(self.ctx.h, M, N, K, batch, alpha, pa, sab, sam, sak, pb, sbb, sbk, sbn,
-                                        beta, pc, scb, scm, scn))
+        if bias is None and pre is None and not gelu:
+            self.ctx._chk(L.atsim_gemm_bf16(self.ctx.h, M, N, K, batch, alpha, pa, sab, sam, sak, pb, sbb, sbk, sbn,
+                                            beta, pc, scb, scm, scn))
+        else:
+            pp = C.c_void_p(pre.data_ptr() + 4 * c_off) if pre is not None else None
+            self.ctx._chk(L.atsim_gemm_bf16_ex(self.ctx.h, M, N, K, batch, alpha, pa, sab, sam, sak, pb, sbb, sbk,
+                                               sbn, beta, pc, scb, scm, scn, _p(bias), pp, int(gelu)))
 
     def add_bias_act(self, a, b, bias, pre, y, Cc, P, gelu):
         n = a.numel()
@@ -167,9 +176,10 @@ class FNO:
         self.Fx, self.Gre, self.Gim, self.Fxi = f(Fx), f(np.cos(ay)), f(-np.sin(ay)), f(Fxi)
 
     # ---------------------------------------------------------------- building blocks
-    def _pointwise(self, Wt, h, out, Cout, Cin, SP, beta=0.0):
-        # out[o][q] = sum_i W[o][i] h[i][q]
-        self.ops.gemm(Cout, SP, Cin, 1, Wt, 0, 0, Cin, 1, h, 0, 0, SP, 1, out, 0, 0, SP, 1, beta=beta)
+    def _pointwise(self, Wt, h, out, Cout, Cin, SP, beta=0.0, bias=None, pre=None, gelu=False):
+        # out[o][q] = act(sum_i W[o][i] h[i][q] + beta out[o][q] + bias[o])
+        self.ops.gemm(Cout, SP, Cin, 1, Wt, 0, 0, Cin, 1, h, 0, 0, SP, 1, out, 0, 0, SP, 1, beta=beta, bias=bias,
+                      pre=pre, gelu=gelu)
 
     def _pointwise_t(self, Wt, dy, out, Cout, Cin, SP, beta=0.0):
         # out[i][q] = sum_o W[o][i] dy[o][q]
@@ -178,7 +188,7 @@ class FNO:
     def _weight_grad(self, dy, h, gW, Cout, Cin, SP):
         # gW[o][i] = sum_q dy[o][q] h[i][q], split over K chunks, partials summed in order
         torch = _torch()
-        kc = 4096 if SP % 4096 == 0 else SP
+        kc = 1024 if SP % 1024 == 0 else SP
         nb = SP // kc
         part = torch.empty((nb, Cout, Cin), dtype=torch.float32, device=self.dev)
         self.ops.gemm(Cout, Cin, kc, nb, dy, 0, kc, SP, 1, h, 0, kc, 1, SP, part, 0, Cout * Cin, Cin, 1)
@@ -280,27 +290,24 @@ class FNO:
         Cw, p, ops = self.width, self.param, self.ops
         cache = {"x": x, "S": S}
         h = torch.empty((Cw, S, self.N, self.N), dtype=torch.float32, device=self.dev)
-        self._pointwise(p["Lw"], x, h, Cw, self.cin, SP)
-        ops.add_bias_act(h, None, p["Lb"], None, h, Cw, SP, False)
+        self._pointwise(p["Lw"], x, h, Cw, self.cin, SP, bias=p["Lb"])
         for b in range(self.blocks):
-            spec = torch.empty_like(h)
-            B = self._spectral(h, p[f"R{b}"], S, spec)
-            lin = torch.empty_like(h)
-            self._pointwise(p[f"W{b}"], h, lin, Cw, Cw, SP)
-            pre = torch.empty_like(h) if keep else None
+            # hn = act(W h + K h + B): the spectral path writes K h, the 1x1 map
+            # adds onto it with its bias and activation in one GEMM epilogue
             hn = torch.empty_like(h)
-            ops.add_bias_act(lin, spec, p[f"B{b}"], pre, hn, Cw, SP, b < self.blocks - 1)
+            B = self._spectral(h, p[f"R{b}"], S, hn)
+            pre = torch.empty_like(h) if keep else None
+            self._pointwise(p[f"W{b}"], h, hn, Cw, Cw, SP, beta=1.0, bias=p[f"B{b}"], pre=pre,
+                            gelu=b < self.blocks - 1)
             if keep:
                 cache[f"h{b}"], cache[f"pre{b}"], cache[f"B{b}"] = h, pre, B
             h = hn
         cache["hL"] = h
         q1 = torch.empty((self.hidden, S, self.N, self.N), dtype=torch.float32, device=self.dev)
         preq = torch.empty_like(q1) if keep else None
-        self._pointwise(p["Q1"], h, q1, self.hidden, Cw, SP)
-        ops.add_bias_act(q1, None, p["q1b"], preq, q1, self.hidden, SP, True)
+        self._pointwise(p["Q1"], h, q1, self.hidden, Cw, SP, bias=p["q1b"], pre=preq, gelu=True)
         y = torch.empty((self.cout, S, self.N, self.N), dtype=torch.float32, device=self.dev)
-        self._pointwise(p["Q2"], q1, y, self.cout, self.hidden, SP)
-        ops.add_bias_act(y, None, p["q2b"], None, y, self.cout, SP, False)
+        self._pointwise(p["Q2"], q1, y, self.cout, self.hidden, SP, bias=p["q2b"])
         if keep:
             cache["preq"], cache["q1"] = preq, q1
             self.cache = cache

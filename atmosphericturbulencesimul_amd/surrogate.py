@@ -4,10 +4,12 @@ and judged serving (no counterpart in the reference).
 Workload (256^2 pupil, the 7-layer profile at 0/0.5/1/2/4/8/16 km with Cn2
 fractions 0.4/0.15/0.1/0.1/0.1/0.1/0.05, r0 = 0.12 m at 0.5 um, 4096^2
 masters, science wavelength 0.7 um):
-* input: the 7 sampled layer screens scaled to phase at 0.7 um;
-* output: pupil log-amplitude chi = log|E| and the diffraction phase
-  dphi = pha - sum_l phase_l (the wrapped difference between the propagated
-  phase and the geometric phase), so the field is exp(chi + i (Phi + dphi)).
+* input: the 7 sampled layer screens scaled to phase at 0.7 um, and their
+  5-point Laplacians (the local term of weak scintillation, finer than the
+  retained modes);
+* output: the smooth complex perturbation u = E exp(-i Phi) of the pupil
+  field about its geometric phase Phi = sum_l phase_l, as (Re u - 1, Im u):
+  amplitude |u|, phase Phi + arg u, with no 2 pi wraps in the target.
 Pixels are 5 mm (1.28 m pupil) so that the Fresnel scales of the upper
 layers, sqrt(lambda z) = 7.5-10.6 cm at 8-16 km, lie inside the 16 retained
 modes; the network cannot represent structure finer than its modes.
@@ -83,15 +85,15 @@ class PairSource:
         self.scr = torch.empty((self.D, self.L, self.P), dtype=torch.float32, device=f"cuda:{ctx.device}")
         self.sc = torch.tensor(np.asarray(scoeff, np.float64), device=f"cuda:{ctx.device}")
         torch.cuda.synchronize()
-        self.scales = (1.0, 1.0, 1.0)
+        self.scales = (1.0, 1.0, 1.0, 1.0)
 
     def fill(self, X, T, Stot, frames, plain=True, s_start=0):
-        """frames simulator steps into samples s_start.. of X [L][Stot][P] (and
+        """frames simulator steps into samples s_start.. of X [2L][Stot][P] (and
         T [2][Stot][P]); X and T must be contiguous.  plain=False only
         advances the layers (no Fresnel frame, no targets)"""
         assert X.is_contiguous() and (T is None or T.is_contiguous())
         L_ = self.ctx.L
-        xs, cs, ps = self.scales
+        xs, xl, cs, ps = self.scales
         for f in range(frames):
             if plain:
                 pha, amp = self.sim.frame(sync=False)
@@ -101,10 +103,10 @@ class PairSource:
             self.ctx._chk(L_.atsim_sim_layer_screens(self.sim.h, self.scr.data_ptr()))
             p = lambda t: t.data_ptr() if t is not None else None
             self.ctx._chk(L_.atsim_nn_make_pairs(self.ctx.h, self.scr.data_ptr(), self.sc.data_ptr(), p(pha),
-                                                 p(amp), self.D, self.L, self.P, X.data_ptr(),
+                                                 p(amp), self.D, self.L, self.N, X.data_ptr(),
                                                  T.data_ptr() if T is not None else None, Stot,
                                                  s_start + f * self.D,
-                                                 xs, cs, ps))
+                                                 xs, xl, cs, ps))
 
 
 def rms(ctx, t):
@@ -127,7 +129,7 @@ def train(ctx, net, src, pairs, frames_per_chunk=32, batch=8, lr=1e-3, epochs_pe
     step = 0
     t0 = time.perf_counter()
     with torch.cuda.stream(ctx.stream):
-        X = torch.empty((L, Sc, P), dtype=torch.float32, device=dev)
+        X = torch.empty((2 * L, Sc, P), dtype=torch.float32, device=dev)
         T = torch.empty((2, Sc, P), dtype=torch.float32, device=dev)
         for ch in range(nchunks):
             src.fill(X, T, Sc, frames_per_chunk)
@@ -155,8 +157,11 @@ def scint_index(amp):
 
 
 def strfn_x(pha, lags):
+    """phase structure function along x of the wrapped phase differences
+    (insensitive to the 2 pi jumps an unwrapped phase carries at branch points)"""
     p = pha.astype(np.float64)
-    return np.array([np.mean((p[:, r:] - p[:, :-r]) ** 2) for r in lags])
+    w = lambda d: np.angle(np.exp(1j * d))
+    return np.array([np.mean(w(p[:, r:] - p[:, :-r]) ** 2) for r in lags])
 
 
 def serve(ctx, net, wl, scales, frames=10000, judge_every=50, fallback_err=0.5, seed=0xA75 + 5100, log=print):
@@ -166,7 +171,7 @@ def serve(ctx, net, wl, scales, frames=10000, judge_every=50, fallback_err=0.5, 
     sim, scoeff = make_sim(ctx, wl, seed, 0.001, [(0.0, 0.0)])
     src = PairSource(ctx, sim, scoeff, wl)
     src.scales = scales
-    xs, cs, ps = scales
+    xs, xl, cs, ps = scales
     P, L, N = src.P, src.L, src.N
     B = judge_every
     lags = [1, 2, 4, 8, 16, 32]
@@ -176,7 +181,7 @@ def serve(ctx, net, wl, scales, frames=10000, judge_every=50, fallback_err=0.5, 
     t_fno = 0.0
     n_fno = 0
     with torch.cuda.stream(ctx.stream):
-        X = torch.empty((L, B, P), dtype=torch.float32, device=dev)
+        X = torch.empty((2 * L, B, P), dtype=torch.float32, device=dev)
         amp_o = torch.empty((B, P), dtype=torch.float32, device=dev)
         pha_o = torch.empty((B, P), dtype=torch.float32, device=dev)
         nblocks = frames // B
@@ -206,7 +211,7 @@ def serve(ctx, net, wl, scales, frames=10000, judge_every=50, fallback_err=0.5, 
             pp = pha_p.cpu().numpy().reshape(N, N)
             af = amp_o[0].cpu().numpy().reshape(N, N)
             pf = pha_o[0].cpu().numpy().reshape(N, N)
-            phi = xs * X[:, 0].sum(0).cpu().numpy().reshape(N, N)
+            phi = xs * X[:L, 0].double().sum(0).cpu().numpy().reshape(N, N)
             Ep = ap * np.exp(1j * pp)
             Ef = af * np.exp(1j * pf)
             Eg = np.exp(1j * phi)
@@ -258,10 +263,10 @@ def fno_rate(ctx, net, wl, scales, frames=500, batch=50, seed=0xA75 + 5300):
     sim, scoeff = make_sim(ctx, wl, seed, 0.001, [(0.0, 0.0)])
     src = PairSource(ctx, sim, scoeff, wl)
     src.scales = scales
-    xs, cs, ps = scales
+    xs, xl, cs, ps = scales
     P, L = src.P, src.L
     with torch.cuda.stream(ctx.stream):
-        X = torch.empty((L, batch, P), dtype=torch.float32, device=dev)
+        X = torch.empty((2 * L, batch, P), dtype=torch.float32, device=dev)
         amp_o = torch.empty((batch, P), dtype=torch.float32, device=dev)
         pha_o = torch.empty((batch, P), dtype=torch.float32, device=dev)
 
@@ -283,19 +288,21 @@ def fno_rate(ctx, net, wl, scales, frames=500, batch=50, seed=0xA75 + 5300):
 
 
 def calibrate(ctx, src, frames=4):
-    """normalisation scales (RMS of input phase, log-amplitude, diffraction phase)"""
+    """normalisation scales: RMS of the input phases, of their Laplacians, and
+    of the real and imaginary parts of the target perturbation"""
     torch = _torch()
     dev = f"cuda:{ctx.device}"
     Sc = frames * src.D
     with torch.cuda.stream(ctx.stream):
-        X = torch.empty((src.L, Sc, src.P), dtype=torch.float32, device=dev)
+        X = torch.empty((2 * src.L, Sc, src.P), dtype=torch.float32, device=dev)
         T = torch.empty((2, Sc, src.P), dtype=torch.float32, device=dev)
-        src.scales = (1.0, 1.0, 1.0)
+        src.scales = (1.0, 1.0, 1.0, 1.0)
         src.fill(X, T, Sc, frames)
-        xs = rms(ctx, X)
+        xs = rms(ctx, X[:src.L])
+        xl = rms(ctx, X[src.L:])
         cs = rms(ctx, T[0])
         ps = rms(ctx, T[1])
-    return (xs, cs, ps)
+    return (xs, xl, cs, ps)
 
 
 def main(argv=None):
@@ -318,8 +325,9 @@ def main(argv=None):
     src = PairSource(ctx, sim, scoeff, wl)
     scales = calibrate(ctx, src)
     src.scales = scales
-    log(f"scales: input phase {scales[0]:.3f} rad, log-amplitude {scales[1]:.4f}, diffraction phase {scales[2]:.4f}")
-    net = FNO(ctx, wl["N"], wl["L"], 2, width=args.width, modes=args.modes, blocks=4, hidden=64, seed=1)
+    log(f"scales: input phase {scales[0]:.3f} rad, Laplacian {scales[1]:.4f}, Re u - 1 {scales[2]:.4f}, "
+        f"Im u {scales[3]:.4f}")
+    net = FNO(ctx, wl["N"], 2 * wl["L"], 2, width=args.width, modes=args.modes, blocks=4, hidden=64, seed=1)
     losses, t_train = train(ctx, net, src, args.pairs, batch=args.batch, lr=args.lr,
                             epochs_per_chunk=args.epochs_per_chunk, log=log)
     sim.close()

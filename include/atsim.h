@@ -204,6 +204,13 @@ int atsim_gemm_bf16(atsim_ctx *ctx, long M, long N, long K, long batch, float al
                     const float *A, long sab, long sam, long sak,
                     const float *B, long sbb, long sbk, long sbn, float beta,
                     float *C, long scb, long scm, long scn);
+/* atsim_gemm_bf16 with an epilogue: v = alpha A B + beta C + bias[m] (bias
+ * may be NULL); pre (may be NULL, C's layout) receives v; C receives GELU(v)
+ * if gelu != 0, else v. */
+int atsim_gemm_bf16_ex(atsim_ctx *ctx, long M, long N, long K, long batch, float alpha,
+                       const float *A, long sab, long sam, long sak,
+                       const float *B, long sbb, long sbk, long sbn, float beta,
+                       float *C, long scb, long scm, long scn, const float *bias, float *pre, int gelu);
 /* y = act(a + b + bias[c]) over n = S*C*P elements laid out [S][C][P]; b, bias
  * and pre may be NULL; pre keeps the activation's input; act = GELU (tanh
  * form) if gelu != 0, else identity. */
@@ -216,16 +223,18 @@ int atsim_nn_bias_grad(atsim_ctx *ctx, const float *dy, float *db, long S, long 
 /* mean squared error; grad (may be NULL) = 2 (pred - target) / n; loss (may
  * be NULL; synchronises the stream) = mean of (pred - target)^2 */
 int atsim_nn_mse(atsim_ctx *ctx, const float *pred, const float *target, float *grad, long n, double *loss);
-/* Pairs of the surrogate from one simulator step: screens [D][L][P] (from
+/* Pairs of the surrogate from one simulator step: screens [D][L][N*N] (from
  * atsim_sim_layer_screens), d_scoeff [L] (device, FP64) the wavelength's
- * phase multipliers; writes for samples s0 .. s0+D-1 of Stot, channel-major:
- * x[l][s][p] = scoeff_l screen_l / xs and, when pha and amp (each [D][P]) are
- * given, t[0][s][p] = log(amp) / cs, t[1][s][p] = (pha - sum_l scoeff_l
- * screen_l) / ps. */
+ * phase multipliers; writes samples s0 .. s0+D-1 of Stot, channel-major
+ * [2L][Stot][N*N]: x[l] = v_l / xs with v_l = scoeff_l screen_l, x[L+l] =
+ * 5-point Laplacian of v_l / xl (edge pixels clamped); when pha and amp (each
+ * [D][N*N]) are given, t [2][Stot][N*N] = ((Re u - 1) / cs, Im u / ps) with
+ * u = amp e^{i (pha - sum_l v_l)}. */
 int atsim_nn_make_pairs(atsim_ctx *ctx, const float *screens, const double *d_scoeff, const float *pha,
-                        const float *amp, long D, long L, long P, float *x, float *t, long Stot, long s0,
-                        double xs, double cs, double ps);
-/* network output y [2][S][P] -> amp = exp(cs y0), pha = xs sum_l x_l + ps y1 */
+                        const float *amp, long D, long L, long N, float *x, float *t, long Stot, long s0,
+                        double xs, double xl, double cs, double ps);
+/* network output y [2][S][P] -> u = (1 + cs y0) + i ps y1, amp = |u|,
+ * pha = xs sum_{l < L} x_l + arg u (x: the network input, phase channels first) */
 int atsim_nn_unpack(atsim_ctx *ctx, const float *y, const float *x, long L, long S, long P, double xs, double cs,
                     double ps, float *amp, float *pha);
 /* out[i] = sum_{b < nb} part[b n + i] (added to out if accumulate) */
