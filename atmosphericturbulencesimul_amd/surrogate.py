@@ -40,10 +40,10 @@ def _torch():
     return torch
 
 
-def workload(dx=0.005):
+def workload(dx=0.005, N=256, M=4096):
     L = 7
     return dict(
-        N=256, M=4096, L=L, dx=dx, lam=0.7e-6, lam_ref=0.5e-6, r0=0.12, L0=20.0, l0=0.01, bin_m=1000.0,
+        N=N, M=M, L=L, dx=dx, lam=0.7e-6, lam_ref=0.5e-6, r0=0.12, L0=20.0, l0=0.01, bin_m=1000.0,
         alts=np.array([0.0, 500.0, 1000.0, 2000.0, 4000.0, 8000.0, 16000.0]),
         cn2=np.array([0.4, 0.15, 0.1, 0.1, 0.1, 0.1, 0.05]),
         speeds=np.linspace(5.0, 40.0, L),
