@@ -114,8 +114,10 @@ __device__ __forceinline__ int xcd_swizzle(int b, int total) {
 // most layers a frame can hold (per-frame layer parameters are staged in LDS)
 constexpr int MAX_LAYERS = 64;
 
-// (long)(x + 1000.5) - 1000, the reference's rounding of unwrap counts
-__device__ __forceinline__ double unwrap_count(double x) { return (double)((long long)(x + 1000.5) - 1000); }
+// (long)(x + 1000.5) - 1000, the reference's rounding of unwrap counts: the
+// C conversion truncates toward zero, which trunc() does exactly in FP64
+// (no 64-bit integer conversion)
+__device__ __forceinline__ double unwrap_count(double x) { return trunc(x + 1000.5) - 1000.0; }
 
 // Raise a kernel's dynamic LDS limit to what the 160 KB of a gfx950 CU leaves
 // beside its static LDS.  Errors are consumed here so that they do not
@@ -755,6 +757,8 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         const int w = k / L, l = k - w * L;
         ss[w][l] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
     }
+    __shared__ int um[8];  // unwrap rule of each wavelength of the chunk
+    if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
     __syncthreads();
     // the fields are needed only after the layers: issue their loads first
     float2 Ein[8];
@@ -817,7 +821,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                 } else {
                     a.amp[oi] = sqrtf(E.x * E.x + E.y * E.y);
                     const double ph = (double)atan2f(E.y, E.x);
-                    if (a.unwrap_mode[w_begin + w] == 0) {
+                    if (um[w] == 0) {
                         a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
                     } else {
                         a.pha[oi] = (float)ph;
@@ -837,7 +841,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         }
         // segmented sum over the wave's lanes (block columns are non-decreasing
         // along the lanes); the last lane of each run adds the run's total
-        if (blockrow && a.unwrap_mode[w_begin + w] != 0) {
+        if (blockrow && um[w] != 0) {
             const int key = ii2;
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
@@ -857,7 +861,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     const int nb = xs * xs;
     for (int k = threadIdx.x; k < Wc * 16; k += EXT_T) {
         const int w = k >> 4, lb = k & 15;
-        if (a.unwrap_mode[w_begin + w] == 0 || b0 + lb >= xs) continue;
+        if (um[w] == 0 || b0 + lb >= xs) continue;
         if (bs[w][0][lb] == 0.0 && bs[w][1][lb] == 0.0) continue;
         const int f = d * a.W + w_begin + w;
         double* dst = a.bsum + ((size_t)f * nb + (size_t)jj2 * xs + b0 + lb) * 2;
@@ -1018,6 +1022,19 @@ hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, flo
     long g = (N * N + 255) / 256;
     dim3 grid((unsigned)(g < 4096 ? g : 4096), (unsigned)L, (unsigned)D);
     k_layer_screens<<<grid, 256, 0, st>>>(masters, ls, out, (int)N, (int)M, (int)L);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fill
+// zero n doubles (a kernel, not a memset, so that it can sit in a captured
+// frame graph)
+__global__ void k_zero_f64(double* __restrict__ p, long n) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0.0;
+}
+
+hipError_t launch_zero_f64(double* p, long n, hipStream_t st) {
+    long g = (n + 255) / 256;
+    k_zero_f64<<<(unsigned)(g < 4096 ? (g > 0 ? g : 1) : 4096), 256, 0, st>>>(p, n);
     return hipGetLastError();
 }
 

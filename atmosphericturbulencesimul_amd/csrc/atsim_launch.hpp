@@ -50,6 +50,7 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* fi
 hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
                               int nfields, int Nout, int xs, hipStream_t st);
 
+hipError_t launch_zero_f64(double* p, long n, hipStream_t st);
 hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, float* out, long N, long M, long L,
                                 long D, hipStream_t st);
 

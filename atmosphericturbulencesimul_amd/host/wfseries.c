@@ -222,6 +222,7 @@ int make_AtmosphericTurbulence_wavefront_series(float slambdaum, long WFprecisio
     d.bicubic = cf.bicubic; d.fresnel = cf.fresnel; d.amplitude = cf.amplitude == 1;
     d.slayer_alt = sa; d.super_layer_index = sli; d.lambda = lam; d.scoeff = scoeff; d.unwrap_mode = unwrap;
     d.xpos0 = x0; d.ypos0 = y0; d.vx = vx; d.vy = vy;
+    if (c) atsim_set_tuning(c, "graph", 1); /* same output buffers every frame: replay the frame's graph */
     int rc = c ? atsim_sim_create(c, &d, &sim) : -1;
     if (rc) {
         atsim_compat_fail(0, "simulation set-up failed: %s", c ? atsim_error(c) : "no GPU");

@@ -50,6 +50,9 @@ int atsim_copy(atsim_ctx *ctx, void *dst, const void *src, size_t nbytes);
 /* Launch-shape tuning of this context (0 restores the built-in default):
  *   "row_mw"  wavelength fields per row-pass workgroup (1..8)
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
+ *   "graph"   1: atsim_sim_frame replays a HIP graph of the frame's kernels,
+ *             captured at the second frame (and again when the output
+ *             pointers change); the position upload stays outside the graph
  * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
 int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
 int atsim_get_tuning(atsim_ctx *ctx, const char *key);
