@@ -75,6 +75,11 @@ _SIGS = {
                                               C.POINTER(C.c_double)]),
     "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atsim_host_alloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t]),
+    "atsim_host_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atsim_copy_async": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "atsim_copy_wait": (C.c_int, [C.c_void_p, C.c_int]),
+    "atsim_stream_wait_copy": (C.c_int, [C.c_void_p, C.c_int]),
     "atsim_sim_advance": (C.c_int, [C.c_void_p]),
     "atsim_gemm_bf16": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_long, C.c_float,
                                   C.c_void_p, C.c_long, C.c_long, C.c_long,

@@ -273,12 +273,17 @@ class Simulation:
             torch.cuda.current_stream(m.device).synchronize()
             self.ctx._chk(self.ctx.L.atsim_sim_set_master(self.h, layer, _ptr(m)))
 
-    def frame(self, field=None, sync=True):
-        """advance one time step; returns (pha, amp) views [D, W, Nout, Nout]"""
-        self.ctx._chk(self.ctx.L.atsim_sim_frame(self.h, _ptr(self.pha), _ptr(self.amp), _ptr(field)))
+    def frame(self, field=None, sync=True, out=None):
+        """advance one time step; returns (pha, amp) [D, W, Nout, Nout], in the
+        simulation's own buffers or in out = (pha, amp) float32 device tensors"""
+        pha, amp = out if out is not None else (self.pha, self.amp)
+        if out is not None:
+            assert pha.numel() == self.pha.numel() and amp.numel() == self.amp.numel()
+            assert pha.dtype == amp.dtype == _torch().float32 and pha.is_contiguous() and amp.is_contiguous()
+        self.ctx._chk(self.ctx.L.atsim_sim_frame(self.h, _ptr(pha), _ptr(amp), _ptr(field)))
         if sync:
             self.ctx.sync()
-        return self.pha, self.amp
+        return pha, amp
 
     def positions(self):
         x = np.empty((self.D, self.L))

@@ -18,6 +18,8 @@
 
 using namespace atsim;
 
+#define ATSIM_COPY_SLOTS 4
+
 struct atsim_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -27,6 +29,10 @@ struct atsim_ctx {
     int row_mw = 0;  // wavelengths per row-pass workgroup
     int col_c = 0;   // columns per column-pass workgroup
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
+    // asynchronous device-to-host copies (default priority, like the main stream)
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ready[ATSIM_COPY_SLOTS] = {};
+    hipEvent_t copy_done[ATSIM_COPY_SLOTS] = {};
     double* nn_part = nullptr;  // 256 partial sums of the network's loss reduction
     float* nn_bias = nullptr;   // per-chunk partial sums of bias gradients
     size_t nn_bias_cap = 0;
@@ -105,6 +111,14 @@ void atsim_ctx_destroy(atsim_ctx* c) {
     for (auto& kv : c->twiddles) hipFree(kv.second);
     if (c->nn_part) hipFree(c->nn_part);
     if (c->nn_bias) hipFree(c->nn_bias);
+    if (c->copy_stream) {
+        hipStreamSynchronize(c->copy_stream);
+        for (int i = 0; i < ATSIM_COPY_SLOTS; i++) {
+            hipEventDestroy(c->copy_ready[i]);
+            hipEventDestroy(c->copy_done[i]);
+        }
+        hipStreamDestroy(c->copy_stream);
+    }
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -143,6 +157,56 @@ int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
     if (!c || !dst || !src) return ATSIM_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(dst, src, nbytes, hipMemcpyDefault));
+    return ATSIM_OK;
+}
+
+int atsim_host_alloc(atsim_ctx* c, void** ptr, size_t nbytes) {
+    if (!c || !ptr) return ATSIM_EINVAL;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, nbytes > 0 ? nbytes : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, ATSIM_ENOMEM, "cannot allocate %zu pinned host bytes", nbytes);
+    }
+    return ATSIM_OK;
+}
+
+int atsim_host_free(atsim_ctx* c, void* ptr) {
+    if (!c) return ATSIM_EINVAL;
+    if (ptr) HIPCHK(c, hipHostFree(ptr));
+    return ATSIM_OK;
+}
+
+static int copy_stream_init(atsim_ctx* c) {
+    if (c->copy_stream) return ATSIM_OK;
+    HIPCHK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    for (int i = 0; i < ATSIM_COPY_SLOTS; i++) {
+        HIPCHK(c, hipEventCreateWithFlags(&c->copy_ready[i], hipEventDisableTiming));
+        HIPCHK(c, hipEventCreateWithFlags(&c->copy_done[i], hipEventDisableTiming));
+    }
+    return ATSIM_OK;
+}
+
+int atsim_copy_async(atsim_ctx* c, int slot, void* dst, const void* src, size_t nbytes) {
+    if (!c || !dst || !src || slot < 0 || slot >= ATSIM_COPY_SLOTS) return ATSIM_EINVAL;
+    if (int r = copy_stream_init(c)) return r;
+    HIPCHK(c, hipEventRecord(c->copy_ready[slot], c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->copy_stream, c->copy_ready[slot], 0));
+    HIPCHK(c, hipMemcpyAsync(dst, src, nbytes, hipMemcpyDefault, c->copy_stream));
+    HIPCHK(c, hipEventRecord(c->copy_done[slot], c->copy_stream));
+    return ATSIM_OK;
+}
+
+int atsim_copy_wait(atsim_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= ATSIM_COPY_SLOTS) return ATSIM_EINVAL;
+    if (!c->copy_stream) return ATSIM_OK;
+    HIPCHK(c, hipEventSynchronize(c->copy_done[slot]));
+    return ATSIM_OK;
+}
+
+int atsim_stream_wait_copy(atsim_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= ATSIM_COPY_SLOTS) return ATSIM_EINVAL;
+    if (!c->copy_stream) return ATSIM_OK;
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->copy_done[slot], 0));
     return ATSIM_OK;
 }
 
@@ -447,13 +511,17 @@ struct atsim_sim {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> prof_recs;  // (kind, index of the first event of the pair)
-    // captured frame graph and the output pointers it was captured with
+    // captured frame graphs, keyed by the output pointers they were captured
+    // with (two sets: double-buffered outputs alternate between them)
     long frames_done = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    float* g_pha = nullptr;
-    float* g_amp = nullptr;
-    void* g_field = nullptr;
+    struct GraphSlot {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        float* pha = nullptr;
+        float* amp = nullptr;
+        void* field = nullptr;
+        long used = -1;
+    } gslot[2];
 };
 
 enum { PROF_ROW = 0, PROF_COL = 1, PROF_EXTRACT = 2, PROF_BLOCK = 3, PROF_KINDS = 4 };
@@ -484,11 +552,14 @@ static void prof_end(atsim_sim* s, long i0) {
     (void)hipEventRecord(s->ev_pool[(size_t)i0 + 1], s->ctx->stream);
 }
 
+static void drop_graph_slot(atsim_sim::GraphSlot& g) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    g = atsim_sim::GraphSlot{};
+}
+
 static void sim_drop_graph(atsim_sim* s) {
-    if (s->gexec) hipGraphExecDestroy(s->gexec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    s->gexec = nullptr;
-    s->graph = nullptr;
+    for (auto& g : s->gslot) drop_graph_slot(g);
 }
 
 static void sim_free(atsim_sim* s) {
@@ -787,8 +858,12 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
         s->frames_done++;
         return enqueue_frame(s, d_pha, d_amp, d_field);
     }
-    if (s->gexec && (s->g_pha != d_pha || s->g_amp != d_amp || s->g_field != d_field)) sim_drop_graph(s);
-    if (!s->gexec) {
+    atsim_sim::GraphSlot* gs = nullptr;
+    for (auto& g : s->gslot)
+        if (g.exec && g.pha == d_pha && g.amp == d_amp && g.field == d_field) gs = &g;
+    if (!gs) {
+        gs = s->gslot[0].used <= s->gslot[1].used ? &s->gslot[0] : &s->gslot[1];
+        drop_graph_slot(*gs);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int r = enqueue_frame(s, d_pha, d_amp, d_field);
         hipGraph_t g = nullptr;
@@ -798,17 +873,18 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
             return r;
         }
         if (e != hipSuccess) return fail(c, ATSIM_EHIP, "frame graph capture: %s", hipGetErrorString(e));
-        s->graph = g;
-        e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+        gs->graph = g;
+        e = hipGraphInstantiate(&gs->exec, g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-            sim_drop_graph(s);
+            drop_graph_slot(*gs);
             return fail(c, ATSIM_EHIP, "frame graph instantiate: %s", hipGetErrorString(e));
         }
-        s->g_pha = d_pha;
-        s->g_amp = d_amp;
-        s->g_field = d_field;
+        gs->pha = d_pha;
+        gs->amp = d_amp;
+        gs->field = d_field;
     }
-    HIPCHK(c, hipGraphLaunch(s->gexec, c->stream));
+    gs->used = s->frames_done;
+    HIPCHK(c, hipGraphLaunch(gs->exec, c->stream));
     s->frames_done++;
     return ATSIM_OK;
 }

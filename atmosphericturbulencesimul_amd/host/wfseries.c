@@ -296,10 +296,18 @@ int make_AtmosphericTurbulence_wavefront_series(float slambdaum, long WFprecisio
     printf("Start TSPAN = %ld\n", start_tspan);
 
     {
-        void *dpha = NULL, *damp = NULL;
-        rc = atsim_malloc(c, &dpha, sizeof(float) * W * no);
-        if (!rc) rc = atsim_malloc(c, &damp, sizeof(float) * W * no);
-        float *hp = (float *)malloc(sizeof(float) * W * no), *ha = (float *)malloc(sizeof(float) * W * no);
+        /* two output sets: frame k computes into set k % 2 while the copy of
+         * frame k-1 travels to pinned host memory on the copy stream; the
+         * host writes cubes and shared memory for frame k-1 meanwhile */
+        const size_t ob = sizeof(float) * W * no;
+        void *dpha[2] = {NULL, NULL}, *damp[2] = {NULL, NULL};
+        float *hp[2] = {NULL, NULL}, *ha[2] = {NULL, NULL};
+        for (int b = 0; b < 2 && !rc; b++) {
+            rc = atsim_malloc(c, &dpha[b], ob);
+            if (!rc) rc = atsim_malloc(c, &damp[b], ob);
+            if (!rc) rc = atsim_host_alloc(c, (void **)&hp[b], ob);
+            if (!rc) rc = atsim_host_alloc(c, (void **)&ha[b], ob);
+        }
         sem_t *waitsem = NULL;
         if (cf.waitforsem == 1) {
             char sn[256];
@@ -313,6 +321,39 @@ int make_AtmosphericTurbulence_wavefront_series(float slambdaum, long WFprecisio
         double scoeff_shm = 1.0; /* AT.c:4286-4296 */
         if (cf.shm_soutputm == 1) scoeff_shm = SLAMBDA / 2.0 / M_PI;
         else if (cf.shm_soutputm == 2) scoeff_shm = SLAMBDA / 2.0 / M_PI * 1e6;
+        long k = 0;                    /* frames launched */
+        long pend_frame = -1, pend_tspan = 0;
+        double pend_t = 0.0;
+        /* host side of the frame in set b: cube slices and shared memory */
+        #define FINISH_PENDING()                                                                          \
+            do {                                                                                          \
+                if (pend_frame >= 0 && !rc) {                                                             \
+                    const int b = (int)((k - 1) & 1);                                                     \
+                    rc = atsim_copy_wait(c, 2 * b);                                                       \
+                    if (!rc) rc = atsim_copy_wait(c, 2 * b + 1);                                          \
+                    if (!rc) {                                                                            \
+                        const float *p_ = hp[b], *a_ = ha[b];                                             \
+                        copy_cube_slice(cube[0], pend_frame, p_, no);                                     \
+                        copy_cube_slice(cube[1], pend_frame, a_, no);                                     \
+                        if (W == 2) {                                                                     \
+                            copy_cube_slice(cube[2], pend_frame, p_ + no, no);                            \
+                            copy_cube_slice(cube[3], pend_frame, a_ + no, no);                            \
+                        }                                                                                 \
+                        if (shm_pha >= 0) {                                                               \
+                            shm_write(shm_pha, p_, no, 1.0, 0, pend_t);                                   \
+                            if (cf.amplitude == 1) shm_write(shm_amp, a_, no, 1.0, 0, pend_t);            \
+                        }                                                                                 \
+                        if (shm_spha >= 0 && W == 2) {                                                    \
+                            shm_write(shm_spha, p_ + no, no, scoeff_shm, 1, pend_t);                      \
+                            if (cf.amplitude == 1) shm_write(shm_samp, a_ + no, no, 1.0, 1, pend_t);      \
+                        }                                                                                 \
+                        printf("\rFrame %4ld/%4ld, File %6ld/%6ld  [TIME = %10.4f s]  ", pend_frame, NBFRAMES, \
+                               pend_tspan, cf.nb_tspan, pend_t);                                          \
+                        fflush(stdout);                                                                   \
+                    }                                                                                     \
+                    pend_frame = -1;                                                                      \
+                }                                                                                         \
+            } while (0)
         for (long tspan = start_tspan; tspan < cf.nb_tspan && !rc; tspan++) {
             for (long frame = 0; frame < NBFRAMES && !rc; frame++) {
                 if (cf.simtdelay > 0) usleep((useconds_t)cf.simtdelay);
@@ -334,27 +375,21 @@ int make_AtmosphericTurbulence_wavefront_series(float slambdaum, long WFprecisio
                     free(px);
                     free(py);
                 }
-                rc = atsim_sim_frame(sim, (float *)dpha, (float *)damp, NULL);
-                if (!rc) rc = atsim_copy(c, hp, dpha, sizeof(float) * W * no);
-                if (!rc) rc = atsim_copy(c, ha, damp, sizeof(float) * W * no);
+                const int b = (int)(k & 1);
+                /* set b was last copied out by frame k - 2 */
+                if (k >= 2) rc = atsim_stream_wait_copy(c, 2 * b);
+                if (!rc && k >= 2) rc = atsim_stream_wait_copy(c, 2 * b + 1);
+                if (!rc) rc = atsim_sim_frame(sim, (float *)dpha[b], (float *)damp[b], NULL);
+                if (!rc) rc = atsim_copy_async(c, 2 * b, hp[b], dpha[b], ob);
+                if (!rc) rc = atsim_copy_async(c, 2 * b + 1, ha[b], damp[b], ob);
                 if (rc) break;
-                copy_cube_slice(cube[0], frame, hp, no);
-                copy_cube_slice(cube[1], frame, ha, no);
-                if (W == 2) {
-                    copy_cube_slice(cube[2], frame, hp + no, no);
-                    copy_cube_slice(cube[3], frame, ha + no, no);
-                }
-                if (shm_pha >= 0) {
-                    shm_write(shm_pha, hp, no, 1.0, 0, t);
-                    if (cf.amplitude == 1) shm_write(shm_amp, ha, no, 1.0, 0, t);
-                }
-                if (shm_spha >= 0 && W == 2) {
-                    shm_write(shm_spha, hp + no, no, scoeff_shm, 1, t);
-                    if (cf.amplitude == 1) shm_write(shm_samp, ha + no, no, 1.0, 1, t);
-                }
-                printf("\rFrame %4ld/%4ld, File %6ld/%6ld  [TIME = %10.4f s]  ", frame, NBFRAMES, tspan, cf.nb_tspan, t);
-                fflush(stdout);
+                FINISH_PENDING();
+                k++;
+                pend_frame = frame;
+                pend_tspan = tspan;
+                pend_t = t;
             }
+            FINISH_PENDING();
             if (rc) break;
             const long lpm = (long)(1.0e12 * SLAMBDA + 0.5);
             char f1[600], f2[600];
@@ -388,12 +423,18 @@ int make_AtmosphericTurbulence_wavefront_series(float slambdaum, long WFprecisio
                 }
             }
         }
+        #undef FINISH_PENDING
         printf("\n");
         if (waitsem) sem_close(waitsem);
-        free(hp);
-        free(ha);
-        atsim_free(c, dpha);
-        atsim_free(c, damp);
+        atsim_sync(c);
+        for (int b = 0; b < 2; b++) {
+            atsim_copy_wait(c, 2 * b);
+            atsim_copy_wait(c, 2 * b + 1);
+            atsim_host_free(c, hp[b]);
+            atsim_host_free(c, ha[b]);
+            atsim_free(c, dpha[b]);
+            atsim_free(c, damp[b]);
+        }
         if (rc) atsim_compat_fail(0, "frame loop failed: %s", atsim_error(c));
     }
 

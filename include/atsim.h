@@ -47,6 +47,18 @@ int atsim_malloc(atsim_ctx *ctx, void **ptr, size_t nbytes);
 int atsim_free(atsim_ctx *ctx, void *ptr);
 /* synchronous copy between host and/or device buffers (unified addressing) */
 int atsim_copy(atsim_ctx *ctx, void *dst, const void *src, size_t nbytes);
+/* Pinned host memory (hipHostMalloc) for asynchronous copies */
+int atsim_host_alloc(atsim_ctx *ctx, void **ptr, size_t nbytes);
+int atsim_host_free(atsim_ctx *ctx, void *ptr);
+/* Asynchronous copy on the context's copy stream, ordered after all work
+ * queued so far on its main stream, so that frame k's outputs travel while
+ * frame k+1 computes.  slot (0..3) names the copy: atsim_copy_wait(slot)
+ * blocks the host until it has landed; atsim_stream_wait_copy(slot) makes
+ * later main-stream work (a frame that overwrites the source) wait for it.
+ * No counterpart in the reference (its frames are copied synchronously). */
+int atsim_copy_async(atsim_ctx *ctx, int slot, void *dst, const void *src, size_t nbytes);
+int atsim_copy_wait(atsim_ctx *ctx, int slot);
+int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
 /* Launch-shape tuning of this context (0 restores the built-in default):
  *   "row_mw"  wavelength fields per row-pass workgroup (1..8)
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
