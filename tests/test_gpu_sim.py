@@ -128,3 +128,12 @@ def test_frame_graph_replays_match_eager(ctx, oracle):
     assert np.array_equal(xe, xg) and np.array_equal(ye, yg)
     gg.close()
     ctx_g.close()
+
+
+@pytest.mark.parametrize("separable", [True, False])
+def test_layer_dependent_phase_multipliers(ctx, oracle, separable):
+    # the reference recomputes Scoeff per layer (AtmosphericTurbulence.c:2661-2663):
+    # multipliers that are, and are not, a wavelength factor times a layer factor
+    sc = [[1.0, 1.0, 1.0], [0.303, 0.303, 0.303]] if separable else [[1.0, 1.0, 1.0], [0.301, 0.305, 0.312]]
+    c = _cfg1_like(oracle, unwrap_mode=[0, 1], scoeff=sc)
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
