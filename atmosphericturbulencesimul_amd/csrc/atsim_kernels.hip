@@ -132,40 +132,65 @@ static void allow_dynamic_lds(const void* kernel) {
 }
 
 
-// Copy nrows x NCOLS master values, rows ys + b and columns xst + c taken
-// modulo M (ys, xst already in [0, M)), into S[r * stride + c] where r runs
-// over the row list.  Each thread issues U loads before it writes any of them
-// to LDS, so the loads of a batch are in flight together.  Row r of the list
-// is (layer q, row b) = (r / 4, r % 4) with per-layer bases ysq[q], xsq[q].
-template <int NCOLS, int U>
-__device__ __forceinline__ void stage_master_rows(float* S, int stride, const float* __restrict__ masters,
-                                                  size_t layer_stride, int M, const int* ysq, const int* xsq,
-                                                  int l0, int nrows, int tid, int nthr) {
-    const int total = nrows * NCOLS;
-    for (int e0 = 0; e0 < total; e0 += U * nthr) {
-        float tmp[U];
+// Master-row staging.  A workgroup samples a layer over one output row
+// segment: it needs the 4 master rows ys..ys+3 and the columns xs-1 .. of its
+// pixels, all taken modulo M.  The row bases are uniform (scalar registers);
+// each lane loads whole columns, so a staged element costs one load, one LDS
+// write and no per-element index arithmetic.
+
+// x mod M for 0 <= x < 2M (the second test only fires for masters narrower
+// than a staged segment)
+__device__ __forceinline__ int wrap_col(int x, int M) {
+    x = x < M ? x : x - M;
+    return x < M ? x : x % M;
+}
+
+// the 4 rows of column x (already wrapped) of layer m, rows ys.. (uniform)
+__device__ __forceinline__ void load_col4(float (&t)[4], const float* __restrict__ m, int M, int ys, int x) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int e = e0 + u * nthr + tid;
-            if (e < total) {
-                const int r = e / NCOLS, c = e - r * NCOLS;
-                const int q = r >> 2, b = r & 3;
-                int y = ysq[q] + b;
-                y = y < M ? y : y - M;
-                int x = xsq[q] + c;
-                x = x < M ? x : x - M;
-                if (x >= M) x = x % M;
-                tmp[u] = masters[(size_t)(l0 + q) * layer_stride + (size_t)y * M + x];
-            }
+    for (int b = 0; b < 4; b++) {
+        int y = ys + b;
+        y = y < M ? y : y - M;
+        t[b] = m[(size_t)y * M + x];
+    }
+}
+
+// columns [0, NC) of the 4 rows of one layer into S[b * stride + c];
+// UNR column slots per lane are loaded before any is written
+template <int UNR>
+__device__ __forceinline__ void stage_rows4(float* S, int stride, const float* __restrict__ m, int M, int ys,
+                                            int xs, int NC, int tid, int nthr) {
+    ys = __builtin_amdgcn_readfirstlane(ys);
+    xs = __builtin_amdgcn_readfirstlane(xs);
+    for (int c0 = tid; c0 < NC; c0 += UNR * nthr) {
+        float t[UNR][4];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const int c = c0 + u * nthr;
+            if (c < NC) load_col4(t[u], m, M, ys, wrap_col(xs + c, M));
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int e = e0 + u * nthr + tid;
-            if (e < total) {
-                const int r = e / NCOLS;
-                S[r * stride + (e - r * NCOLS)] = tmp[u];
+        for (int u = 0; u < UNR; u++) {
+            const int c = c0 + u * nthr;
+            if (c < NC) {
+#pragma unroll
+                for (int b = 0; b < 4; b++) S[b * stride + c] = t[u][b];
             }
         }
+    }
+}
+
+// tail columns [NC0, NC0 + nt) of the 4 rows of nl layers (row bases ysq[q],
+// column bases xsq[q]), one element per lane: S[(q * 4 + b) * stride + c]
+__device__ __forceinline__ void stage_tail(float* S, int stride, const float* __restrict__ masters,
+                                           size_t layer_stride, int M, const int* ysq, const int* xsq, int nl,
+                                           int NC0, int nt, int tid, int nthr) {
+    for (int e = tid; e < nl * 4 * nt; e += nthr) {
+        const int r = e / nt, c = NC0 + (e - r * nt);
+        const int q = r >> 2, b = r & 3;
+        int y = ysq[q] + b;
+        y = y < M ? y : y - M;
+        S[r * stride + c] = masters[(size_t)q * layer_stride + (size_t)y * M + wrap_col(xsq[q] + c, M)];
     }
 }
 
@@ -309,7 +334,8 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
         for (int seg0 = 0; seg0 < N; seg0 += SEG) {
             __syncthreads();
             const int ys = mod_m(p.y0 + j - 1, M), xst = mod_m(p.x0 + seg0 - 1, M);  // uniform
-            stage_master_rows<SEG + 3, 9>(S, SW, m, 0, M, &ys, &xst, 0, 4, threadIdx.x, nthr);
+            stage_rows4<2>(S, SW, m, M, ys, xst, SEG, threadIdx.x, nthr);
+            stage_tail(S, SW, m, 0, M, &ys, &xst, 1, SEG, 3, threadIdx.x, nthr);
             __syncthreads();
             for (int i = threadIdx.x; i < SEG; i += nthr) {
                 double v = 0.0;
@@ -773,8 +799,31 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     for (int l0 = 0; l0 < L; l0 += EXT_LC) {
         const int nl = min(EXT_LC, L - l0);
         __syncthreads();
-        stage_master_rows<EXT_T + 3, 21>(&S[0][0][0], EXT_T + 4, a.all.masters, (size_t)M * M, M, &ysq[l0],
-                                         &xsq[l0], l0, nl * 4, threadIdx.x, EXT_T);
+        {
+            // main columns: lane c stages column c of the 4 rows of EXT_G layers at a time
+            constexpr int EXT_G = 5;
+            for (int q0 = 0; q0 < nl; q0 += EXT_G) {
+                float t[EXT_G][4];
+#pragma unroll
+                for (int g = 0; g < EXT_G; g++) {
+                    const int q = q0 + g;
+                    if (q < nl) {
+                        const int ys = __builtin_amdgcn_readfirstlane(ysq[l0 + q]);
+                        const int xs0 = __builtin_amdgcn_readfirstlane(xsq[l0 + q]);
+                        load_col4(t[g], a.all.masters + (size_t)(l0 + q) * M * M, M, ys,
+                                  wrap_col(xs0 + (int)threadIdx.x, M));
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < EXT_G; g++)
+                    if (q0 + g < nl) {
+#pragma unroll
+                        for (int b = 0; b < 4; b++) S[q0 + g][b][threadIdx.x] = t[g][b];
+                    }
+            }
+            stage_tail(&S[0][0][0], EXT_T + 4, a.all.masters + (size_t)l0 * M * M, (size_t)M * M, M, &ysq[l0],
+                       &xsq[l0], nl, EXT_T, 3, threadIdx.x, EXT_T);
+        }
         __syncthreads();
         if (valid) {
             for (int q = 0; q < nl; q++) {
