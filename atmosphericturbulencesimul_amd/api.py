@@ -292,6 +292,14 @@ class Simulation:
                                                      y.ctypes.data_as(C.POINTER(C.c_double))))
         return x, y
 
+    def block_sums(self):
+        """(D, W, xs, xs, 2) float64 block sums (sum cos, sum sin of PHI - pha)
+        of the last frame's science-lambda rule, xs = Nout // 20"""
+        xs = self.Nout // 20
+        out = np.zeros((self.D, self.W, xs, xs, 2))
+        self.ctx._chk(self.ctx.L.atsim_sim_block_sums(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
     def gaps(self):
         return self.ctx.L.atsim_sim_gaps(self.h)
 

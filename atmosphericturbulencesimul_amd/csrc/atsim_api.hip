@@ -959,6 +959,21 @@ int atsim_sim_profile_read(atsim_sim* s, int kind, double* ms, long* count) {
 }
 
 // ------------------------------------------------------------------ layer screens
+int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
+    if (!s || !out) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long F = s->D * s->W, need = F * s->xs * s->xs * 2;
+    if (n != need) return fail(c, ATSIM_EINVAL, "block sums: n = %ld, expected %ld", n, need);
+    if (need == 0) return ATSIM_OK;
+    if (s->nblockfields == 0 || !s->bsum) {
+        memset(out, 0, sizeof(double) * need);
+        return ATSIM_OK;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, s->bsum, sizeof(double) * need, hipMemcpyDeviceToHost));
+    return ATSIM_OK;
+}
+
 int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {
     if (!s || !d_out) return ATSIM_EINVAL;
     atsim_ctx* c = s->ctx;

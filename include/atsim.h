@@ -196,6 +196,15 @@ long atsim_sim_gaps(atsim_sim *sim);
 int atsim_sim_profile(atsim_sim *sim, int enable);
 int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
 
+/* Block sums of the science-lambda unwrap rule of the last frame: out [D][W]
+ * [xs][xs][2] (host, n = D W xs xs 2 doubles, xs = Nout / 20) holds, per
+ * block (jj2, ii2), sum cos(PHI - pha) and sum sin(PHI - pha) over its pixels
+ * -- the bre / bim sums of AtmosphericTurbulence.c:3523-3625, before the
+ * block fix-up; zero for fields of the reference-lambda rule.  Synchronises
+ * the context stream.  A diagnostic for tests, with no entry point in the
+ * reference. */
+int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
+
 /* Advance every layer by one time step without computing a frame (the
  * positions a following atsim_sim_layer_screens samples).  No counterpart in
  * the reference: the surrogate's frames skip the layer loop. */

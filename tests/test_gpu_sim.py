@@ -137,3 +137,43 @@ def test_layer_dependent_phase_multipliers(ctx, oracle, separable):
     sc = [[1.0, 1.0, 1.0], [0.303, 0.303, 0.303]] if separable else [[1.0, 1.0, 1.0], [0.301, 0.305, 0.312]]
     c = _cfg1_like(oracle, unwrap_mode=[0, 1], scoeff=sc)
     _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
+
+
+def _block_sums_np(E, PHI, Nout):
+    """sum cos / sin of PHI - arg E per (jj2, ii2) block over the centred
+    Nout x Nout crop, blocks (long)(1.0 * i / Nout * xs) with xs = Nout / 20
+    (AtmosphericTurbulence.c:3523-3625)"""
+    N = E.shape[0]
+    o = (N - Nout) // 2
+    xs = Nout // 20
+    d = PHI[o:o + Nout, o:o + Nout] - np.angle(E[o:o + Nout, o:o + Nout])
+    k = (np.arange(Nout) / Nout * xs).astype(np.int64)
+    bre = np.zeros((xs, xs))
+    bim = np.zeros((xs, xs))
+    np.add.at(bre, (k[:, None], k[None, :]), np.cos(d))
+    np.add.at(bim, (k[:, None], k[None, :]), np.sin(d))
+    return bre, bim
+
+
+def test_block_sums_match_reference_sums(ctx, oracle):
+    # two row segments per output row (Nout > 256), blocks of 20 and 21
+    # pixels (Nout = 490, xs = 24), two directions, two science-rule fields
+    c = _cfg1_like(oracle, N=512, Nout=490, M=1024, lambdas=[0.5e-6, 1.25e-6, 2.2e-6], unwrap_mode=[0, 1, 1],
+                   directions=((0.0, 0.0), (2e-6, -1e-6)))
+    osims = oracle_sims(oracle, c)
+    gs = gpu_sim(ctx, c)
+    field = torch.empty((c.D, c.W, c.N, c.N), dtype=torch.complex64, device="cuda")
+    for _ in range(2):
+        gs.frame(field)
+        bs = gs.block_sums()
+        for d, os_ in enumerate(osims):
+            E_o, PHI_o, _, _ = os_.frame()
+            for w in range(c.W):
+                if c.unwrap[w] == 0:
+                    assert not bs[d, w].any()
+                    continue
+                bre, bim = _block_sums_np(E_o[w], PHI_o[w], c.Nout)
+                # a pixel left out or counted twice moves a sum by ~1
+                err = max(np.abs(bs[d, w, :, :, 0] - bre).max(), np.abs(bs[d, w, :, :, 1] - bim).max())
+                print("block sums", d, w, err, np.abs(bre).max())
+                assert err < 2e-3
