@@ -15,6 +15,7 @@ The functions mirror the reference's operators for the wavefront path:
 Tensors are torch CUDA tensors (complex64 fields, float32 screens/outputs).
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -44,9 +45,12 @@ class Context:
             raise AtsimError(f"atsim_ctx_create failed ({rc})")
         self.h = h
         self._torch_stream = torch.cuda.ExternalStream(self.L.atsim_stream(h), device=f"cuda:{device}")
+        self._sims = weakref.WeakSet()  # simulations on this context, closed before it
 
     def close(self):
         if getattr(self, "h", None):
+            for s in list(getattr(self, "_sims", ())):
+                s.close()
             self.L.atsim_ctx_destroy(self.h)
             self.h = None
 
@@ -241,6 +245,7 @@ class Simulation:
         h = C.c_void_p()
         ctx._chk(ctx.L.atsim_sim_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        ctx._sims.add(self)
         self.N, self.Nout, self.M, self.L, self.W, self.D = N, Nout, M, L, W, D
         torch = _torch()
         dev = f"cuda:{ctx.device}"
@@ -248,9 +253,10 @@ class Simulation:
         self.amp = torch.empty_like(self.pha)
 
     def close(self):
-        if getattr(self, "h", None):
+        # a simulation outliving its context was already closed by it
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.ctx.L.atsim_sim_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

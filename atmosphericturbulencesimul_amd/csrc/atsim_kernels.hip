@@ -730,6 +730,25 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
     return hipGetLastError();
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave: shifts by 1, 2, 4, 8
+// lanes within each 16-lane row, then row 0's and row 2's last lane added to
+// rows 1 and 3 (row_bcast:15) and lane 31 added to rows 2 and 3
+// (row_bcast:31).  Lanes a shift leaves without a source add zero.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_shift_add(float x) {
+    const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false);
+    return x + __int_as_float(y);
+}
+__device__ __forceinline__ float wave_inclusive_sum(float x) {
+    x = dpp_shift_add<0x111, 0xf>(x);  // row_shr:1
+    x = dpp_shift_add<0x112, 0xf>(x);  // row_shr:2
+    x = dpp_shift_add<0x114, 0xf>(x);  // row_shr:4
+    x = dpp_shift_add<0x118, 0xf>(x);  // row_shr:8
+    x = dpp_shift_add<0x142, 0xa>(x);  // row_bcast:15
+    x = dpp_shift_add<0x143, 0xc>(x);  // row_bcast:31
+    return x;
+}
+
 // ------------------------------------------------------------------ extraction
 // One workgroup per (256-pixel row segment, row, direction x wavelength
 // chunk of <= 8), one pixel per thread.  For every layer the 4 master rows
@@ -853,6 +872,11 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     }
     const double inv2pi = 0.15915494309189533577;
     const int lane = threadIdx.x & 63;
+    // block column of lane + 1 (-1: none, or the last lane) and whether this
+    // lane ends a run of one block column
+    int next_key = __shfl_down(ii2, 1, 64);
+    const bool run_end = ii2 >= 0 && (lane == 63 || next_key != ii2);
+    if (lane == 63 || next_key == ii2) next_key = -1;
 #pragma unroll
     for (int w = 0; w < 8; w++) {
         if (w >= Wc) continue;
@@ -888,20 +912,22 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                 }
             }
         }
-        // segmented sum over the wave's lanes (block columns are non-decreasing
-        // along the lanes); the last lane of each run adds the run's total
+        // per-block sums over the wave: with P the wave's inclusive prefix sum,
+        // a run of lanes s..e of one block sums to P[e] - P[s-1].  Block
+        // columns are non-decreasing along the valid lanes and lanes outside
+        // every block add zero, so the last lane e of each run adds P[e] to its
+        // block and subtracts it from the block of lane e + 1.
         if (blockrow && um[w] != 0) {
-            const int key = ii2;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const float ux = __shfl_up(cx, off, 64), uy = __shfl_up(cy, off, 64);
-                const int uk = __shfl_up(key, off, 64);
-                if (lane >= off && uk == key) { cx += ux; cy += uy; }
-            }
-            const int nk = __shfl_down(key, 1, 64);
-            if (key >= 0 && (lane == 63 || nk != key) && key - b0 < 16) {
-                atomicAdd(&bs[w][0][key - b0], (double)cx);
-                atomicAdd(&bs[w][1][key - b0], (double)cy);
+            const double px = (double)wave_inclusive_sum(cx), py = (double)wave_inclusive_sum(cy);
+            if (run_end) {
+                if (ii2 - b0 < 16) {
+                    atomicAdd(&bs[w][0][ii2 - b0], px);
+                    atomicAdd(&bs[w][1][ii2 - b0], py);
+                }
+                if (next_key >= 0 && next_key - b0 < 16) {
+                    atomicAdd(&bs[w][0][next_key - b0], -px);
+                    atomicAdd(&bs[w][1][next_key - b0], -py);
+                }
             }
         }
     }

@@ -177,3 +177,17 @@ def test_block_sums_match_reference_sums(ctx, oracle):
                 err = max(np.abs(bs[d, w, :, :, 0] - bre).max(), np.abs(bs[d, w, :, :, 1] - bim).max())
                 print("block sums", d, w, err, np.abs(bre).max())
                 assert err < 2e-3
+
+
+def test_context_closed_before_its_simulation(oracle):
+    # the context closes the simulations still open on it; a later close or
+    # garbage collection of the simulation must not touch the freed context
+    from atmosphericturbulencesimul_amd.api import Context
+    c = _cfg1_like(oracle)
+    ctx2 = Context(0)
+    gs = gpu_sim(ctx2, c)
+    gs.frame()
+    ctx2.close()
+    assert gs.h is None
+    gs.close()
+    del gs
