@@ -28,6 +28,7 @@ struct atsim_ctx {
     // launch-shape tuning (0 = built-in default)
     int row_mw = 0;  // wavelengths per row-pass workgroup
     int col_c = 0;   // columns per column-pass workgroup
+    int col_e = 0;   // elements per thread of the column Fresnel pass (0 = default, 16, 32)
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
@@ -219,6 +220,9 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
         if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
             return fail(c, ATSIM_EINVAL, "col_c must be 0, 2, 4, 8 or 16");
         c->col_c = value;
+    } else if (!strcmp(key, "col_e")) {
+        if (value != 0 && value != 16 && value != 32) return fail(c, ATSIM_EINVAL, "col_e must be 0, 16 or 32");
+        c->col_e = value;
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "graph must be 0 or 1");
         c->graph = value;
@@ -232,6 +236,7 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!c || !key) return ATSIM_EINVAL;
     if (!strcmp(key, "row_mw")) return c->row_mw;
     if (!strcmp(key, "col_c")) return c->col_c;
+    if (!strcmp(key, "col_e")) return c->col_e;
     if (!strcmp(key, "graph")) return c->graph;
     return ATSIM_EINVAL;
 }
@@ -805,7 +810,8 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
             HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, c->stream, c->row_mw));
             prof_end(s, pr);
             long pc = prof_begin(s, PROF_COL);
-            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream, c->col_c));
+            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream, c->col_c,
+                                 c->col_e));
             prof_end(s, pc);
         }
     }

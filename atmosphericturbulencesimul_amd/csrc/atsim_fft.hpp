@@ -164,12 +164,14 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) 
 
 // one stage's twiddle + butterfly on registers.  The R-1 twiddles w^r of a
 // butterfly are composed from the table values w^1, w^2, w^4, w^8 (at most
-// three complex products each) instead of R-1 table loads.
-template <int N, int R, int NS, int DIR>
+// three complex products each) instead of R-1 table loads.  E is the number
+// of elements a thread holds (16 or 32): T = N / E threads per transform,
+// E / R butterflies per thread.
+template <int N, int R, int NS, int DIR, int E = 16>
 __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __restrict__ tw) {
-    constexpr int T = N / 16;
+    constexpr int T = N / E;
 #pragma unroll
-    for (int b = 0; b < 16 / R; b++) {
+    for (int b = 0; b < E / R; b++) {
         if constexpr (NS > 1) {
             const int j = t + b * T;
             const int k = j % NS;
@@ -192,29 +194,49 @@ __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __
             }
         }
         dftR<R, DIR>(v + b * R);
+        // with 32 elements per thread, one butterfly at a time keeps the
+        // temporaries of a single butterfly live
+        if constexpr (E > 16) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-template <int N, int R, int NS, int C>
+// Padded LDS positions as one per-thread base plus compile-time offsets
+// (immediate offsets of the LDS instructions, no per-element address
+// registers).  For every plan here (all N, E = 16 or 32, every stage) the pad
+// of a store position P0 + r NS is pad(P0) + r NS / 16, and the pad of a
+// load position t + m (m = b T + r N / R) is pad(t) + m / 16: the carries
+// never cross a 16-element boundary.
+template <int N, int R, int NS, int E>
+__device__ __forceinline__ int store_base(int t, int b) {
+    constexpr int T = N / E;
+    const int j = t + b * T;
+    const int p0 = (j / NS) * NS * R + j % NS;
+    return p0 + (p0 >> 4);
+}
+template <int R, int NS>
+__device__ __forceinline__ constexpr int store_off(int r) { return r * NS + (r * NS) / 16; }
+template <int N, int R, int E>
+__device__ __forceinline__ constexpr int load_off(int b, int r) {
+    return (b * (N / E) + r * (N / R)) + (b * (N / E) + r * (N / R)) / 16;
+}
+
+template <int N, int R, int NS, int C, int E = 16>
 __device__ __forceinline__ void stage_store(const float2* v, float2* lds, int t, int c) {
-    constexpr int T = N / 16;
 #pragma unroll
-    for (int b = 0; b < 16 / R; b++) {
-        const int j = t + b * T;
-        const int k = j % NS;
-        const int base = (j / NS) * NS * R + k;
+    for (int b = 0; b < E / R; b++) {
+        float2* p = lds + store_base<N, R, NS, E>(t, b) * C + c;
 #pragma unroll
-        for (int r = 0; r < R; r++) lds[lds_addr<C>(base + r * NS, c)] = v[b * R + r];
+        for (int r = 0; r < R; r++) p[store_off<R, NS>(r) * C] = v[b * R + r];
     }
 }
 
-template <int N, int R, int C>
+template <int N, int R, int C, int E = 16>
 __device__ __forceinline__ void stage_load(float2* v, const float2* lds, int t, int c) {
-    constexpr int T = N / 16;
+    const float2* p = lds + (t + (t >> 4)) * C + c;
 #pragma unroll
-    for (int b = 0; b < 16 / R; b++)
+    for (int b = 0; b < E / R; b++)
 #pragma unroll
-        for (int r = 0; r < R; r++) v[b * R + r] = lds[lds_addr<C>(t + b * T + r * (N / R), c)];
+        for (int r = 0; r < R; r++) v[b * R + r] = p[load_off<N, R, E>(b, r) * C];
 }
 
 // Full transform on registers.  All threads of the workgroup must call it
@@ -247,9 +269,68 @@ __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, int c, c
 }
 
 // index held by register slot q = b*R + r in the layout of a stage of radix R
-template <int N, int R>
+template <int N, int R, int E = 16>
 __device__ __forceinline__ int reg_index(int t, int q) {
-    return t + (q / R) * (N / 16) + (q % R) * (N / R);
+    return t + (q / R) * (N / E) + (q % R) * (N / R);
+}
+
+// The same exchange through an LDS array of floats, one component at a time
+// (real parts, then imaginary parts): half the LDS of a float2 exchange for
+// two more barriers per stage.
+template <int N, int R, int NS, int C, int E, int PART>
+__device__ __forceinline__ void stage_store_part(const float2* v, float* lds, int t, int c) {
+#pragma unroll
+    for (int b = 0; b < E / R; b++) {
+        float* p = lds + store_base<N, R, NS, E>(t, b) * C + c;
+#pragma unroll
+        for (int r = 0; r < R; r++) p[store_off<R, NS>(r) * C] = PART ? v[b * R + r].y : v[b * R + r].x;
+    }
+}
+
+template <int N, int R, int C, int E, int PART>
+__device__ __forceinline__ void stage_load_part(float2* v, const float* lds, int t, int c) {
+    const float* p = lds + (t + (t >> 4)) * C + c;
+#pragma unroll
+    for (int b = 0; b < E / R; b++)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float x = p[load_off<N, R, E>(b, r) * C];
+            if (PART) v[b * R + r].y = x;
+            else v[b * R + r].x = x;
+        }
+}
+
+template <int N, int DIR, int C, int E, int S>
+__device__ __forceinline__ void exchange_split(float2* v, float* lds, int t, int c) {
+    using P = Plan<N>;
+    __syncthreads();
+    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 0>(v, lds, t, c);
+    __syncthreads();
+    stage_load_part<N, P::radix(S + 1, DIR), C, E, 0>(v, lds, t, c);
+    __syncthreads();
+    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 1>(v, lds, t, c);
+    __syncthreads();
+    stage_load_part<N, P::radix(S + 1, DIR), C, E, 1>(v, lds, t, c);
+}
+
+// Full transform on registers, E elements per thread, exchanges through a
+// float LDS region of C interleaved transforms (C * Plan<N>::LDS floats).
+template <int N, int DIR, int C, int E>
+__device__ __forceinline__ void fft_regs_split(float2* v, float* lds, int t, int c, const float2* __restrict__ tw) {
+    using P = Plan<N>;
+    stage_compute<N, P::radix(0, DIR), P::ns(0, DIR), DIR, E>(v, t, tw);
+    if constexpr (P::S > 1) {
+        exchange_split<N, DIR, C, E, 0>(v, lds, t, c);
+        stage_compute<N, P::radix(1, DIR), P::ns(1, DIR), DIR, E>(v, t, tw);
+    }
+    if constexpr (P::S > 2) {
+        exchange_split<N, DIR, C, E, 1>(v, lds, t, c);
+        stage_compute<N, P::radix(2, DIR), P::ns(2, DIR), DIR, E>(v, t, tw);
+    }
+    if constexpr (P::S > 3) {
+        exchange_split<N, DIR, C, E, 2>(v, lds, t, c);
+        stage_compute<N, P::radix(3, DIR), P::ns(3, DIR), DIR, E>(v, t, tw);
+    }
 }
 
 }  // namespace atsim

@@ -111,6 +111,24 @@ __device__ __forceinline__ int xcd_swizzle(int b, int total) {
     return ((((r >> 2) << 3) | xcd) << 2) | (r & 3);
 }
 
+// Raw buffer access (vector memory): a resource over [p, p + bytes), a
+// per-lane byte offset and a scalar byte offset.  Accesses past `bytes`
+// read zero and drop the store.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const u32x2 x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return make_float2(__uint_as_float(x.x), __uint_as_float(x.y));
+}
+__device__ __forceinline__ void buf_store2(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    u32x2 x;
+    x.x = __float_as_uint(v.x);
+    x.y = __float_as_uint(v.y);
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, voff, soff, 0);
+}
+
 // most layers a frame can hold (per-frame layer parameters are staged in LDS)
 constexpr int MAX_LAYERS = 64;
 
@@ -279,6 +297,38 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
     for (int q = 0; q < 16; q++) base[(size_t)reg_index<N, RO>(t, q) * N] = v[q];
 }
 
+// Column Fresnel pass with E = 32 elements per thread (N / 32 threads per
+// column, C * N / 32 per workgroup) and float exchanges: C columns take
+// C * (N + N/16) * 4 B of LDS.  C = 8 (68 KB at N = 2048) lets two
+// workgroups share a CU; C = 16 (136 KB) reads 128-byte row segments.  At
+// most 128 VGPRs (4 waves per SIMD).
+template <int N, int C, int E>
+__global__ void __launch_bounds__(C * (N / E)) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_split(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw) {
+    using P = Plan<N>;
+    constexpr int RI = P::first(-1), RO = P::last(+1), RL = P::last(-1);
+    __shared__ float lds[C * P::LDS];
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int ntile = N / C;
+    const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int f = tile / ntile;
+    const int col = (tile - f * ntile) * C + c;
+    // buffer addressing: one lane offset (row t, column col) and the
+    // register slot's row as a constant scalar offset
+    const auto fr = buf_rsrc(data + (size_t)f * N * N, (uint32_t)N * N * 8);
+    const uint32_t lo = ((uint32_t)t * N + col) * 8;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = buf_load2(fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
+    fft_regs_split<N, -1, C, E>(v, lds, t, c, tw);
+    const auto hr = buf_rsrc(tf + (size_t)(f % W) * N, (uint32_t)N * 8);
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t * 8, reg_index<N, RL, E>(0, q) * 8));
+    fft_regs_split<N, +1, C, E>(v, lds, t, c, tw);
+#pragma unroll
+    for (int q = 0; q < E; q++) buf_store2(v[q], fr, lo, (reg_index<N, RO, E>(0, q) * N) * 8);
+}
+
 // Fresnel x pass for up to MW wavelength fields of the same direction and
 // spatial row j: the screen phase of the layer group is sampled once per
 // pixel (FP64, master rows staged in LDS) and shared by the wavelengths,
@@ -431,7 +481,19 @@ static int col_default_c() { return ColCfg<N>::C; }
 
 template <int N>
 static hipError_t col_dispatch(int op, int dir, float2* data, long F, const float2* tf, int W, const float2* tw,
-                               int c, hipStream_t st) {
+                               int c, int e, hipStream_t st) {
+    if constexpr (N >= 256 && N <= 2048) {
+        if (e == 32 && op == OP_FRESNEL && (c <= 0 || c == 8)) {
+            dim3 grid(N / 8, (unsigned)F), block(8 * (N / 32));
+            k_col_split<N, 8, 32><<<grid, block, 0, st>>>(data, tf, W, tw);
+            return hipGetLastError();
+        }
+        if (e == 32 && op == OP_FRESNEL && c == 16) {
+            dim3 grid(N / 16, (unsigned)F), block(16 * (N / 32));
+            k_col_split<N, 16, 32><<<grid, block, 0, st>>>(data, tf, W, tw);
+            return hipGetLastError();
+        }
+    }
     if (c <= 0) c = col_default_c<N>();
     switch (c) {
         case 2: return col_launch<N, 2>(op, dir, data, F, tf, W, tw, st);
@@ -470,10 +532,10 @@ hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float
 }
 
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
-                      const float2* tw, hipStream_t st, int c) {
+                      const float2* tw, hipStream_t st, int c, int e) {
     switch (n) {
 #define X(NN) \
-    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, st);
+    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, e, st);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;

@@ -62,6 +62,8 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
 /* Launch-shape tuning of this context (0 restores the built-in default):
  *   "row_mw"  wavelength fields per row-pass workgroup (1..8)
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
+ *   "col_e"   elements per thread of the column Fresnel pass (16, or 32:
+ *             8 columns per workgroup exchanging through half the LDS)
  *   "graph"   1: atsim_sim_frame replays a HIP graph of the frame's kernels,
  *             captured at the second frame (and again when the output
  *             pointers change); the position upload stays outside the graph

@@ -191,3 +191,17 @@ def test_context_closed_before_its_simulation(oracle):
     assert gs.h is None
     gs.close()
     del gs
+
+
+@pytest.mark.parametrize("N,cols", [(256, 8), (512, 8), (256, 16), (512, 16)])
+def test_column_pass_32_elements_per_thread(oracle, N, cols):
+    # column Fresnel pass with 32 elements per thread and float LDS exchanges,
+    # 8 or 16 columns per workgroup
+    from atmosphericturbulencesimul_amd.api import Context
+    ctx2 = Context(0)
+    ctx2.set_tuning("col_e", 32)
+    ctx2.set_tuning("col_c", cols)
+    assert ctx2.get_tuning("col_e") == 32
+    c = _cfg1_like(oracle, N=N, Nout=N - 16, M=1024, unwrap_mode=[0, 1])
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx2, c), nframes=2)
+    ctx2.close()
