@@ -792,6 +792,32 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
     return hipGetLastError();
 }
 
+// atan2(y, x) in FP32: t = min(|x|, |y|) / max(|x|, |y|) in [0, 1] with the
+// hardware reciprocal, atan t = t P(t^2) with a degree-7 polynomial in t^2
+// (least-squares fit of the absolute error on [0, 1], written here; 1.3e-7
+// rad in FP32 Horner form), then the octant: pi/2 - a when |y| > |x|, pi - a
+// when x < 0, and the sign of y.  Total error below 4e-7 rad (the float
+// spacing near pi is 2.4e-7; float atan2 itself errs by up to 3.4e-7 on
+// random Gaussian arguments); (0, 0) -> 0.
+__device__ __forceinline__ float atan2_poly(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+    const float s = t * t;
+    float p = -0.004054518882185221f;
+    p = fmaf(p, s, 0.021862762048840523f);
+    p = fmaf(p, s, -0.05591200292110443f);
+    p = fmaf(p, s, 0.09642169624567032f);
+    p = fmaf(p, s, -0.13908617198467255f);
+    p = fmaf(p, s, 0.19946561753749847f);
+    p = fmaf(p, s, -0.33329859375953674f);
+    p = fmaf(p, s, 0.9999993443489075f);
+    float r = p * t;
+    if (ay > ax) r = 1.5707963267948966f - r;
+    if (x < 0.f) r = 3.141592653589793f - r;
+    return copysignf(r, y);
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave: shifts by 1, 2, 4, 8
 // lanes within each 16-lane row, then row 0's and row 2's last lane added to
 // rows 1 and 3 (row_bcast:15) and lane 31 added to rows 2 and 3
@@ -824,7 +850,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     __shared__ float S[EXT_LC][4][EXT_T + 4];
     __shared__ double bs[8][2][16];
     __shared__ LayerSample sls[MAX_LAYERS];  // this direction's layer samples
-    __shared__ double ss[8][MAX_LAYERS];     // phase scales of this chunk's wavelengths
+    __shared__ double ss[MAX_LAYERS][8];     // phase scales of this chunk's wavelengths (0 past Wc)
     const int nchunks = (a.W + 7) / 8;
     const int seg = blockIdx.x, row = blockIdx.y;
     const int d = blockIdx.z / nchunks, chunk = blockIdx.z - d * nchunks;
@@ -862,7 +888,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     }
     for (int k = threadIdx.x; k < 8 * L; k += EXT_T) {
         const int w = k / L, l = k - w * L;
-        ss[w][l] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
+        ss[l][w] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
     }
     __shared__ int um[8];  // unwrap rule of each wavelength of the chunk
     if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
@@ -920,14 +946,18 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                     h = fma(p.wx[3], (double)r[3], h);
                     v = fma(p.wy[b], h, v);
                 }
-                const bool inlast = l >= a.last_first && l < a.last_first + a.last_count;
+                // wavelengths past Wc have zero scales: no per-wavelength tests
+                const double* sl = ss[l];
+                if (l >= a.last_first && l < a.last_first + a.last_count) {  // uniform
 #pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    if (w < Wc) {
-                        const double sv = ss[w][l] * v;
+                    for (int w = 0; w < 8; w++) {
+                        const double sv = sl[w] * v;
                         PHI[w] += sv;
-                        if (inlast) PL[w] += sv;
+                        PL[w] += sv;
                     }
+                } else {
+#pragma unroll
+                    for (int w = 0; w < 8; w++) PHI[w] += sl[w] * v;
                 }
             }
         }
@@ -939,24 +969,35 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     int next_key = __shfl_down(ii2, 1, 64);
     const bool run_end = ii2 >= 0 && (lane == 63 || next_key != ii2);
     if (lane == 63 || next_key == ii2) next_key = -1;
+    // unwrap rules of the chunk as a uniform bit mask (bit w: block rule)
+    int umask = 0;
+#pragma unroll
+    for (int w = 0; w < 8; w++) umask |= (um[w] != 0) << w;
+    umask = __builtin_amdgcn_readfirstlane(umask);
+    // output element of the chunk's first field; the next field is a plane further
+    const size_t plane = (size_t)a.Nout * a.Nout;
+    const size_t o0 = ((size_t)(d * a.W + w_begin) * a.Nout + (rowin ? jj : 0)) * a.Nout + (colin ? ii : 0);
 #pragma unroll
     for (int w = 0; w < 8; w++) {
         if (w >= Wc) continue;
         const int f = d * a.W + w_begin + w;
+        const bool block_rule = (umask >> w) & 1;
         float cx = 0.f, cy = 0.f;  // this pixel's contribution to its block sums
         if (valid) {
             float2 E = Ein[w];
             if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
             if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
             if (rowin && colin) {
-                const size_t oi = ((size_t)f * a.Nout + jj) * a.Nout + ii;
+                const size_t oi = o0 + (size_t)w * plane;
                 if (!a.amplitude) {
                     a.pha[oi] = (float)PHI[w];
                     a.amp[oi] = 1.f;
                 } else {
-                    a.amp[oi] = sqrtf(E.x * E.x + E.y * E.y);
-                    const double ph = (double)atan2f(E.y, E.x);
-                    if (um[w] == 0) {
+                    // |E| with the hardware square root (1 ulp); arg E with
+                    // the polynomial arctangent (< 4e-7 rad)
+                    a.amp[oi] = __builtin_amdgcn_sqrtf(E.x * E.x + E.y * E.y);
+                    const double ph = (double)atan2_poly(E.y, E.x);
+                    if (!block_rule) {
                         a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
                     } else {
                         a.pha[oi] = (float)ph;
@@ -979,7 +1020,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         // columns are non-decreasing along the valid lanes and lanes outside
         // every block add zero, so the last lane e of each run adds P[e] to its
         // block and subtracts it from the block of lane e + 1.
-        if (blockrow && um[w] != 0) {
+        if (blockrow && block_rule) {
             const double px = (double)wave_inclusive_sum(cx), py = (double)wave_inclusive_sum(cy);
             if (run_end) {
                 if (ii2 - b0 < 16) {
