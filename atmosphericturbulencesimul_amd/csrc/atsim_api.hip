@@ -504,7 +504,8 @@ struct atsim_sim {
     static constexpr int RING = 8;
     double* s_dev = nullptr;
     int* unwrap_dev = nullptr;
-    double* dsave = nullptr;
+    float* dres = nullptr;   // [F][Nout][Nout] block-rule remainders
+    int* blk = nullptr;      // [Nout] block index of each output column / row
     double* bsum = nullptr;
     double* bin = nullptr;
     double* ch = nullptr;
@@ -582,7 +583,8 @@ static void sim_free(atsim_sim* s) {
     }
     hipFree(s->s_dev);
     hipFree(s->unwrap_dev);
-    hipFree(s->dsave);
+    hipFree(s->dres);
+    hipFree(s->blk);
     hipFree(s->bsum);
     hipFree(s->bin);
     hipFree(s->ch);
@@ -656,6 +658,7 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     SIMALLOC(s->ls_dev, sizeof(LayerSample) * s->D * s->L);
     SIMALLOC(s->s_dev, sizeof(double) * s->W * s->L);
     SIMALLOC(s->unwrap_dev, sizeof(int) * s->W);
+    SIMALLOC(s->blk, sizeof(int) * (s->Nout > 0 ? s->Nout : 1));
     if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
     if (r == ATSIM_OK) {
@@ -670,7 +673,7 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     s->xs = (int)(s->Nout / 20);
     if (s->nblockfields > 0) {
         const size_t nb = (size_t)(s->xs > 0 ? s->xs * s->xs : 1);
-        SIMALLOC(s->dsave, sizeof(double) * F * no);
+        SIMALLOC(s->dres, sizeof(float) * F * no);
         SIMALLOC(s->bsum, sizeof(double) * F * nb * 2);
         SIMALLOC(s->bin, sizeof(double) * F * nb);
         SIMALLOC(s->ch, sizeof(double) * F * nb);
@@ -688,9 +691,13 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         if (hipMemcpy(s->tf, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
             r = fail(c, ATSIM_EHIP, "sim: table upload failed");
     }
+    // block of output column / row i: (long)(1.0 * i / Nout * xs) (AT.c:3611)
+    std::vector<int> blk(s->Nout > 0 ? s->Nout : 1, 0);
+    for (long i = 0; i < s->Nout; i++) blk[i] = (int)((double)i / s->Nout * s->xs);
     if (r == ATSIM_OK) {
         if (hipMemcpy(s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(s->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice) != hipSuccess ||
             (s->nblockfields > 0 &&
              hipMemcpy(s->flist_dev, flist.data(), sizeof(int) * flist.size(), hipMemcpyHostToDevice) != hipSuccess))
             r = fail(c, ATSIM_EHIP, "sim: parameter upload failed");
@@ -820,7 +827,8 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     a.field_out = (float2*)d_field;
     a.pha = d_pha;
     a.amp = d_amp;
-    a.dsave = s->dsave;
+    a.dres = s->dres;
+    a.blk = s->blk;
     a.bsum = s->bsum;
     a.unwrap_mode = s->unwrap_dev;
     a.all = sa;
@@ -846,8 +854,8 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     if (s->nblockfields > 0) {
         long pb = prof_begin(s, PROF_BLOCK);
         HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->flist_dev, s->nblockfields, s->xs, c->stream));
-        HIPCHK(c, launch_block_apply(d_pha, s->dsave, s->bin, s->flist_dev, s->nblockfields, (int)s->Nout, s->xs,
-                                     c->stream));
+        HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields, (int)s->Nout,
+                                     s->xs, c->stream));
         prof_end(s, pb);
     }
     return ATSIM_OK;

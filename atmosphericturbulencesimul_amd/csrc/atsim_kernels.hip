@@ -874,7 +874,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     const int ii_lo = max(xfirst - o, 0);
     const int b0 = (xs > 0 && ii_lo < a.Nout) ? (int)((double)ii_lo / a.Nout * xs) : 0;
     // block column of this pixel (AT.c:3611: (long)(1.0*ii/Nout*xs)); -1 if it adds to no block
-    const int ii2 = (blockrow && colin) ? (int)((double)ii / a.Nout * xs) : -1;
+    const int ii2 = (blockrow && colin) ? a.blk[ii] : -1;
     const int L = a.all.L;
     const int M = (int)a.all.M;
     // per-frame layer parameters, read once into LDS (global loads of uniform
@@ -1000,16 +1000,21 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                     if (!block_rule) {
                         a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
                     } else {
-                        a.pha[oi] = (float)ph;
-                        if (a.dsave) a.dsave[oi] = PHI[w] - ph;
+                        // whole turns of PHI - atan2 go into the phase now, the
+                        // FP64 remainder (rounded to float) waits for the
+                        // block's peak; the final count of the block rule is
+                        // n + count((r - peak) / 2 pi)
+                        const double dd = PHI[w] - ph;
+                        const double n = rint(dd * inv2pi);
+                        const double r = fma(-6.283185307179586477, n, dd);
+                        a.pha[oi] = (float)fma(6.283185307179586477, n, ph);
+                        if (a.dres) a.dres[oi] = (float)r;
                         if (ii2 >= 0) {
-                            // contribution to the block's mean direction: FP64
-                            // reduction to [-pi, pi], hardware sin/cos (the
-                            // block sums only feed the atan2 of a 400-pixel sum)
-                            const double dd = PHI[w] - ph;
-                            const float r = (float)(dd - 6.283185307179586477 * rint(dd * inv2pi));
-                            cx = __cosf(r);
-                            cy = __sinf(r);
+                            // contribution to the block's mean direction,
+                            // hardware sin/cos (the block sums only feed the
+                            // atan2 of a 400-pixel sum)
+                            cx = __cosf((float)r);
+                            cy = __sinf((float)r);
                         }
                     }
                 }
@@ -1151,31 +1156,47 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* fi
     return hipGetLastError();
 }
 
-__global__ void k_block_apply(float* __restrict__ pha, const double* __restrict__ dsave,
-                              const double* __restrict__ bin, const int* __restrict__ flist, int Nout, int xs) {
+// pha = atan2 + 2 pi (n + count((r - peak) / 2 pi)) with the block's peak
+// (AT.c:4240-4262: count((PHI - atan2 - peak) / 2 pi), the whole turns n of
+// PHI - atan2 already in pha).  One output row of one field per workgroup.
+__global__ void __launch_bounds__(256) k_block_apply(float* __restrict__ pha, const float* __restrict__ dres,
+                                                     const double* __restrict__ bin, const int* __restrict__ flist,
+                                                     const int* __restrict__ blk, int Nout, int xs) {
     const int f = flist[blockIdx.y];
-    const long no = (long)Nout * Nout;
-    const int nb = xs * xs;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < no; e += (long)gridDim.x * blockDim.x) {
-        const int jj = (int)(e / Nout), ii = (int)(e - (long)jj * Nout);
-        double peak = 0.0;
-        if (xs > 0) {
-            const int ii2 = (int)((double)ii / Nout * xs), jj2 = (int)((double)jj / Nout * xs);
-            if (ii2 < xs && jj2 < xs) peak = bin[(size_t)f * nb + jj2 * xs + ii2];
+    const int jj = blockIdx.x;
+    // every pixel lies in a block ((long)(1.0 i / Nout xs) < xs); no blocks: peak 0
+    const double* brow = bin + ((size_t)f * xs + (xs > 0 ? blk[jj] : 0)) * xs;
+    float* prow = pha + ((size_t)f * Nout + jj) * Nout;
+    const float* rrow = dres + ((size_t)f * Nout + jj) * Nout;
+    // r is a float-rounded remainder, so (r - peak) / (2 pi) as a product with
+    // 1 / (2 pi) changes no decision that the rounding of r leaves alone
+    const double inv2pi = 0.15915494309189533577;
+    if ((Nout & 3) == 0 && (reinterpret_cast<uintptr_t>(pha) & 15) == 0) {  // 16-byte rows
+        for (int i4 = threadIdx.x; i4 < Nout / 4; i4 += 256) {
+            const int4 b = reinterpret_cast<const int4*>(blk)[i4];
+            const float4 r = reinterpret_cast<const float4*>(rrow)[i4];
+            float4 p = reinterpret_cast<float4*>(prow)[i4];
+            const double pk0 = xs > 0 ? brow[b.x] : 0.0, pk1 = xs > 0 ? brow[b.y] : 0.0;
+            const double pk2 = xs > 0 ? brow[b.z] : 0.0, pk3 = xs > 0 ? brow[b.w] : 0.0;
+            p.x = (float)fma(2.0 * M_PI, unwrap_count(((double)r.x - pk0) * inv2pi), (double)p.x);
+            p.y = (float)fma(2.0 * M_PI, unwrap_count(((double)r.y - pk1) * inv2pi), (double)p.y);
+            p.z = (float)fma(2.0 * M_PI, unwrap_count(((double)r.z - pk2) * inv2pi), (double)p.z);
+            p.w = (float)fma(2.0 * M_PI, unwrap_count(((double)r.w - pk3) * inv2pi), (double)p.w);
+            reinterpret_cast<float4*>(prow)[i4] = p;
         }
-        const size_t oi = (size_t)f * no + e;
-        const double p = pha[oi];
-        pha[oi] = (float)(p + 2.0 * M_PI * unwrap_count((dsave[oi] - peak) / 2.0 / M_PI));
+        return;
+    }
+    for (int ii = threadIdx.x; ii < Nout; ii += 256) {
+        const double peak = xs > 0 ? brow[blk[ii]] : 0.0;
+        prow[ii] = (float)fma(2.0 * M_PI, unwrap_count(((double)rrow[ii] - peak) * inv2pi), (double)prow[ii]);
     }
 }
 
-hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
-                              int nfields, int Nout, int xs, hipStream_t st) {
-    if (nfields <= 0) return hipSuccess;
-    long no = (long)Nout * Nout;
-    int bx = (int)((no + 255) / 256 < 4096 ? (no + 255) / 256 : 4096);
-    dim3 grid(bx, nfields);
-    k_block_apply<<<grid, 256, 0, st>>>(pha, dsave, bin, field_list, Nout, xs);
+hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,
+                              const int* blk, int nfields, int Nout, int xs, hipStream_t st) {
+    if (nfields <= 0 || Nout <= 0) return hipSuccess;
+    dim3 grid(Nout, nfields);
+    k_block_apply<<<grid, 256, 0, st>>>(pha, dres, bin, field_list, blk, Nout, xs);
     return hipGetLastError();
 }
 

@@ -33,7 +33,10 @@ struct ExtractArgs {
     float2* field_out;       // [F][N][N] or null
     float* pha;              // [F][Nout][Nout]
     float* amp;              // [F][Nout][Nout]
-    double* dsave;           // [F][Nout][Nout] PHI - atan2 for block-rule fields (or null)
+    // block-rule fields: pha holds atan2 + 2 pi n and dres the remainder
+    // r = PHI - atan2 - 2 pi n in [-pi, pi], n = rint((PHI - atan2) / 2 pi)
+    float* dres;             // [F][Nout][Nout] (or null)
+    const int* blk;          // [Nout] block index (long)(1.0 * i / Nout * xs) of column / row i
     double* bsum;            // [F][nb][2] block sums of cos/sin(PHI - atan2)
     const int* unwrap_mode;  // [W]
     ScreenArgs all;          // every layer: the geometric phase PHI

@@ -48,8 +48,8 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
 hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* field_list, int nfields,
                             int xs, hipStream_t st);
-hipError_t launch_block_apply(float* pha, const double* dsave, const double* bin, const int* field_list,
-                              int nfields, int Nout, int xs, hipStream_t st);
+hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,
+                              const int* blk, int nfields, int Nout, int xs, hipStream_t st);
 
 hipError_t launch_zero_f64(double* p, long n, hipStream_t st);
 hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, float* out, long N, long M, long L,

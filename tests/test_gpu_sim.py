@@ -205,3 +205,11 @@ def test_column_pass_32_elements_per_thread(oracle, N, cols):
     c = _cfg1_like(oracle, N=N, Nout=N - 16, M=1024, unwrap_mode=[0, 1])
     _compare(c, oracle_sims(oracle, c), gpu_sim(ctx2, c), nframes=2)
     ctx2.close()
+
+
+@pytest.mark.parametrize("Nout", [102, 121])
+def test_block_rule_output_not_a_multiple_of_four(ctx, oracle, Nout):
+    # block-rule unwrap over rows that the four-pixel path of the final pass
+    # does not take (Nout % 4 != 0), blocks of 20 and 21 pixels
+    c = _cfg1_like(oracle, N=128, Nout=Nout, M=256, unwrap_mode=[0, 1])
+    _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
