@@ -1,6 +1,7 @@
 // atsim_api.hip -- host runtime and C ABI (include/atsim.h) of the wavefront
 // library: contexts, device buffers, transfer-function tables, the per-frame
 // launch sequence of the layer loop.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -29,6 +30,7 @@ struct atsim_ctx {
     int row_mw = 0;  // wavelengths per row-pass workgroup
     int col_c = 0;   // columns per column-pass workgroup
     int col_e = 0;   // elements per thread of the column Fresnel pass (0 = default, 16, 32)
+    int streams = 2; // 2..4: the Fresnel passes of that many wavelength sets run on as many streams (1: one)
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
@@ -223,6 +225,9 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else if (!strcmp(key, "col_e")) {
         if (value != 0 && value != 16 && value != 32) return fail(c, ATSIM_EINVAL, "col_e must be 0, 16 or 32");
         c->col_e = value;
+    } else if (!strcmp(key, "streams")) {
+        if (value < 0 || value > 4) return fail(c, ATSIM_EINVAL, "streams must be in [0, 4]");
+        c->streams = value == 0 ? 2 : value;
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "graph must be 0 or 1");
         c->graph = value;
@@ -237,6 +242,7 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!strcmp(key, "row_mw")) return c->row_mw;
     if (!strcmp(key, "col_c")) return c->col_c;
     if (!strcmp(key, "col_e")) return c->col_e;
+    if (!strcmp(key, "streams")) return c->streams;
     if (!strcmp(key, "graph")) return c->graph;
     return ATSIM_EINVAL;
 }
@@ -504,6 +510,10 @@ struct atsim_sim {
     static constexpr int RING = 8;
     double* s_dev = nullptr;
     int* unwrap_dev = nullptr;
+    // extra streams of the multi-stream Fresnel passes (set k > 0 on xs_stream[k - 1])
+    static constexpr int MAX_SETS = 4;
+    hipStream_t xs_stream[MAX_SETS - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_first[MAX_SETS - 1] = {}, ev_join[MAX_SETS - 1] = {};
     float* dres = nullptr;   // [F][Nout][Nout] block-rule remainders
     int* blk = nullptr;      // [Nout] block index of each output column / row
     double* bsum = nullptr;
@@ -584,6 +594,12 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->s_dev);
     hipFree(s->unwrap_dev);
     hipFree(s->dres);
+    for (int k = 0; k < atsim_sim::MAX_SETS - 1; k++) {
+        if (s->xs_stream[k]) hipStreamDestroy(s->xs_stream[k]);
+        if (s->ev_first[k]) hipEventDestroy(s->ev_first[k]);
+        if (s->ev_join[k]) hipEventDestroy(s->ev_join[k]);
+    }
+    if (s->ev_fork) hipEventDestroy(s->ev_fork);
     hipFree(s->blk);
     hipFree(s->bsum);
     hipFree(s->bin);
@@ -661,6 +677,13 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     SIMALLOC(s->blk, sizeof(int) * (s->Nout > 0 ? s->Nout : 1));
     if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
+    if (r == ATSIM_OK && hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "sim: event creation failed");
+    for (int k = 0; r == ATSIM_OK && k < atsim_sim::MAX_SETS - 1; k++)
+        if (hipStreamCreateWithFlags(&s->xs_stream[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_first[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_join[k], hipEventDisableTiming) != hipSuccess)
+            r = fail(c, ATSIM_EHIP, "sim: stream / event creation failed");
     if (r == ATSIM_OK) {
         s->ls_ev = new hipEvent_t[atsim_sim::RING];
         for (int i = 0; i < atsim_sim::RING; i++) hipEventCreateWithFlags(&s->ls_ev[i], hipEventDisableTiming);
@@ -808,7 +831,38 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     sa.W = (int)W;
     const long F = D * W;
     const long G = (long)s->groups.size() - 1;
-    if (s->propagate) {
+    const int nsets = (int)std::min<long>(c->streams, W);
+    if (s->propagate && nsets >= 2 && !s->prof) {
+        // nsets wavelength sets, each its own chain of row / column passes:
+        // set 0 on the context stream, set k on xs_stream[k - 1], started
+        // after set k - 1's first row pass, so that one set's row pass
+        // (arithmetic-bound) runs beside another's column pass
+        // (memory-bound).  All join the context stream before extraction.
+        HIPCHK(c, hipEventRecord(s->ev_fork, c->stream));
+        for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
+        for (long g = 0; g < G; g++) {
+            sa.lfirst = s->groups[g].lfirst;
+            sa.lcount = s->groups[g].lcount;
+            const float2* tab = s->tf + (size_t)g * W * s->N;
+            for (int k = 0; k < nsets; k++) {
+                hipStream_t st = k ? s->xs_stream[k - 1] : c->stream;
+                sa.wlo = (int)(W * k / nsets);
+                sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
+                HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, st, c->row_mw));
+                if (g == 0 && k + 1 < nsets) {
+                    HIPCHK(c, hipEventRecord(s->ev_first[k], st));
+                    HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k], s->ev_first[k], 0));
+                }
+                HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, D * sa.wcnt, tab, (int)W, s->tw, st, c->col_c,
+                                     c->col_e, sa.wlo, sa.wcnt));
+            }
+        }
+        sa.wlo = sa.wcnt = 0;
+        for (int k = 1; k < nsets; k++) {
+            HIPCHK(c, hipEventRecord(s->ev_join[k - 1], s->xs_stream[k - 1]));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, s->ev_join[k - 1], 0));
+        }
+    } else if (s->propagate) {
         for (long g = 0; g < G; g++) {
             sa.lfirst = s->groups[g].lfirst;
             sa.lcount = s->groups[g].lcount;

@@ -269,7 +269,8 @@ k_row(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
 
 template <int N, int OP, int DIR, int C>
 __global__ void __launch_bounds__(C * (N / 16))
-k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw) {
+k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw, int wlo,
+      int wcnt) {
     using P = Plan<N>;
     constexpr int RI = (OP == OP_FRESNEL) ? P::first(-1) : P::first(DIR);
     constexpr int RO = (OP == OP_FRESNEL) ? P::last(+1) : P::last(DIR);
@@ -277,8 +278,10 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int ntile = N / C;
     const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int f = tile / ntile;
-    const int col = (tile - f * ntile) * C + c;
+    const int fi = tile / ntile;
+    // field of launch index fi: all fields, or wavelengths wlo.. of every direction
+    const int f = wcnt > 0 ? (fi / wcnt) * W + wlo + fi % wcnt : fi;
+    const int col = (tile - fi * ntile) * C + c;
     float2* base = data + (size_t)f * N * N + col;
     float2 v[16];
 #pragma unroll
@@ -304,15 +307,17 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
 // most 128 VGPRs (4 waves per SIMD).
 template <int N, int C, int E>
 __global__ void __launch_bounds__(C * (N / E)) __attribute__((amdgpu_waves_per_eu(4)))
-k_col_split(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw) {
+k_col_split(float2* __restrict__ data, const float2* __restrict__ tf, int W, const float2* __restrict__ tw, int wlo,
+            int wcnt) {
     using P = Plan<N>;
     constexpr int RI = P::first(-1), RO = P::last(+1), RL = P::last(-1);
     __shared__ float lds[C * P::LDS];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int ntile = N / C;
     const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int f = tile / ntile;
-    const int col = (tile - f * ntile) * C + c;
+    const int fi = tile / ntile;
+    const int f = wcnt > 0 ? (fi / wcnt) * W + wlo + fi % wcnt : fi;
+    const int col = (tile - fi * ntile) * C + c;
     // buffer addressing: one lane offset (row t, column col) and the
     // register slot's row as a constant scalar offset
     const auto fr = buf_rsrc(data + (size_t)f * N * N, (uint32_t)N * N * 8);
@@ -353,11 +358,12 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     float* S = reinterpret_cast<float*>(lds);            // [4][SW] staged master rows
     double* V = reinterpret_cast<double*>(S + 4 * SW);  // [SEG] sampled screen
     const int mwc = blockDim.x / T;                      // wavelengths per workgroup (<= MW)
-    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, w0 = blockIdx.z * mwc;
-    const int nw = min(mwc, sa.W - w0);
+    const int wend = sa.wcnt > 0 ? sa.wlo + sa.wcnt : sa.W;  // end of this launch's wavelength range
+    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, w0 = (sa.wcnt > 0 ? sa.wlo : 0) + blockIdx.z * mwc;
+    const int nw = min(mwc, wend - w0);
     const int lw = threadIdx.x / T, t = threadIdx.x - lw * T;
     const bool active = lw < nw;
-    const int w = min(w0 + lw, sa.W - 1);
+    const int w = min(w0 + lw, wend - 1);
     const int f = d * sa.W + w;
     const int nthr = blockDim.x;
     const int M = (int)sa.M;
@@ -371,7 +377,7 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     for (int k = threadIdx.x; k < sa.lcount; k += nthr) sls[k] = sa.ls[d * sa.L + sa.lfirst + k];
     for (int k = threadIdx.x; k < mwc * sa.lcount; k += nthr) {
         const int lw2 = k / sa.lcount, l = k - lw2 * sa.lcount;
-        ss[lw2][l] = sa.s[min(w0 + lw2, sa.W - 1) * sa.L + sa.lfirst + l];
+        ss[lw2][l] = sa.s[min(w0 + lw2, wend - 1) * sa.L + sa.lfirst + l];
     }
     // each layer of the group multiplies the field by its own e^{i s v}: no
     // per-element phase accumulators stay live across the staging
@@ -425,11 +431,12 @@ static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const floa
                                        const ScreenArgs& sa, int mw, hipStream_t st) {
     constexpr int MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG;
     int mwc = (mw > 0 && mw < MW) ? mw : MW;
-    if (sa.W < mwc) mwc = sa.W;
+    const int nwl = sa.wcnt > 0 ? sa.wcnt : sa.W;  // wavelengths of this launch
+    if (nwl < mwc) mwc = nwl;
     size_t lds = (size_t)mwc * Plan<N>::LDS * sizeof(float2);
     const size_t staging = (size_t)16 * (SEG + 4) + (size_t)8 * SEG;
     if (lds < staging) lds = staging;
-    dim3 grid(N, D, (sa.W + mwc - 1) / mwc), block(mwc * (N / 16));
+    dim3 grid(N, D, (nwl + mwc - 1) / mwc), block(mwc * (N / 16));
     static bool attr = false;
     if (!attr) {
         allow_dynamic_lds((const void*)k_row_fresnel<N, true>);
@@ -464,14 +471,14 @@ constexpr bool col_ok() {
 
 template <int N, int C>
 static hipError_t col_launch(int op, int dir, float2* data, long F, const float2* tf, int W, const float2* tw,
-                             hipStream_t st) {
+                             int wlo, int wcnt, hipStream_t st) {
     if constexpr (!col_ok<N, C>()) {
         return hipErrorInvalidValue;
     } else {
         dim3 grid(N / C, (unsigned)F), block(C * (N / 16));
-        if (op == OP_FRESNEL) k_col<N, OP_FRESNEL, 0, C><<<grid, block, 0, st>>>(data, tf, W, tw);
-        else if (dir < 0) k_col<N, OP_FFT, -1, C><<<grid, block, 0, st>>>(data, tf, W, tw);
-        else k_col<N, OP_FFT, +1, C><<<grid, block, 0, st>>>(data, tf, W, tw);
+        if (op == OP_FRESNEL) k_col<N, OP_FRESNEL, 0, C><<<grid, block, 0, st>>>(data, tf, W, tw, wlo, wcnt);
+        else if (dir < 0) k_col<N, OP_FFT, -1, C><<<grid, block, 0, st>>>(data, tf, W, tw, 0, 0);
+        else k_col<N, OP_FFT, +1, C><<<grid, block, 0, st>>>(data, tf, W, tw, 0, 0);
         return hipGetLastError();
     }
 }
@@ -481,25 +488,25 @@ static int col_default_c() { return ColCfg<N>::C; }
 
 template <int N>
 static hipError_t col_dispatch(int op, int dir, float2* data, long F, const float2* tf, int W, const float2* tw,
-                               int c, int e, hipStream_t st) {
+                               int c, int e, int wlo, int wcnt, hipStream_t st) {
     if constexpr (N >= 256 && N <= 2048) {
         if (e == 32 && op == OP_FRESNEL && (c <= 0 || c == 8)) {
             dim3 grid(N / 8, (unsigned)F), block(8 * (N / 32));
-            k_col_split<N, 8, 32><<<grid, block, 0, st>>>(data, tf, W, tw);
+            k_col_split<N, 8, 32><<<grid, block, 0, st>>>(data, tf, W, tw, wlo, wcnt);
             return hipGetLastError();
         }
         if (e == 32 && op == OP_FRESNEL && c == 16) {
             dim3 grid(N / 16, (unsigned)F), block(16 * (N / 32));
-            k_col_split<N, 16, 32><<<grid, block, 0, st>>>(data, tf, W, tw);
+            k_col_split<N, 16, 32><<<grid, block, 0, st>>>(data, tf, W, tw, wlo, wcnt);
             return hipGetLastError();
         }
     }
     if (c <= 0) c = col_default_c<N>();
     switch (c) {
-        case 2: return col_launch<N, 2>(op, dir, data, F, tf, W, tw, st);
-        case 4: return col_launch<N, 4>(op, dir, data, F, tf, W, tw, st);
-        case 8: return col_launch<N, 8>(op, dir, data, F, tf, W, tw, st);
-        case 16: return col_launch<N, 16>(op, dir, data, F, tf, W, tw, st);
+        case 2: return col_launch<N, 2>(op, dir, data, F, tf, W, tw, wlo, wcnt, st);
+        case 4: return col_launch<N, 4>(op, dir, data, F, tf, W, tw, wlo, wcnt, st);
+        case 8: return col_launch<N, 8>(op, dir, data, F, tf, W, tw, wlo, wcnt, st);
+        case 16: return col_launch<N, 16>(op, dir, data, F, tf, W, tw, wlo, wcnt, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -532,10 +539,10 @@ hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float
 }
 
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
-                      const float2* tw, hipStream_t st, int c, int e) {
+                      const float2* tw, hipStream_t st, int c, int e, int wlo, int wcnt) {
     switch (n) {
 #define X(NN) \
-    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, e, st);
+    case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, e, wlo, wcnt, st);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;

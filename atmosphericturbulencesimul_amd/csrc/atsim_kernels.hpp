@@ -26,6 +26,8 @@ struct ScreenArgs {
     const double* s;         // W x L
     long M;
     int L, W, lfirst, lcount;
+    // wavelengths wlo .. wlo + wcnt - 1 of every direction (wcnt 0: all W)
+    int wlo, wcnt;
 };
 
 struct ExtractArgs {

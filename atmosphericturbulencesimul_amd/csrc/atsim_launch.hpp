@@ -24,10 +24,11 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
                               const ScreenArgs& sa, hipStream_t st, int mw = 0);
 // Column pass: same transforms along y; no screen.
-// c: columns per workgroup (0 = default for n); e = 32: the Fresnel pass
+// F fields, or (wcnt > 0) F = D * wcnt fields: wavelengths wlo .. wlo + wcnt - 1
+// of each direction.  c: columns per workgroup (0 = default for n); e = 32: the Fresnel pass
 // holds 32 elements per thread and exchanges through float LDS (256 <= n <= 2048)
 hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float2* tf, int W,
-                      const float2* tw, hipStream_t st, int c = 0, int e = 0);
+                      const float2* tw, hipStream_t st, int c = 0, int e = 0, int wlo = 0, int wcnt = 0);
 
 hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st);
 hipError_t launch_noise(float2* out, long n, uint64_t seed, uint32_t stream, hipStream_t st);

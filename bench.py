@@ -156,6 +156,8 @@ def main():
                     "wavelength, at most 16)")
     ap.add_argument("--row-mw", type=int, default=0, help="wavelengths per row-pass workgroup (0 = default)")
     ap.add_argument("--col-c", type=int, default=0, help="columns per column-pass workgroup (0 = default)")
+    ap.add_argument("--streams", type=int, default=0, help="Fresnel passes of this many wavelength sets on as many "
+                    "streams (0 = default, 2; 1 = one stream)")
     ap.add_argument("--col-e", type=int, default=0, help="elements per thread of the column pass (0 = default, "
                     "16, 32)")
     ap.add_argument("--graph", type=int, default=1, help="1: frames replay a captured HIP graph of their kernels "
@@ -184,6 +186,8 @@ def main():
         ctx.set_tuning("col_c", args.col_c)
     if args.col_e:
         ctx.set_tuning("col_e", args.col_e)
+    if args.streams:
+        ctx.set_tuning("streams", args.streams)
     if args.graph:
         ctx.set_tuning("graph", args.graph)
     if args.calib_copy_mib > 0:
@@ -355,7 +359,7 @@ def main():
         "batched_vs_per_wavelength": round(value / per_lambda["value"], 2) if per_lambda else None,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c"),
-                   "col_e": ctx.get_tuning("col_e"),
+                   "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),
                    "graph": ctx.get_tuning("graph")},
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
     }

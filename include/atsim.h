@@ -64,6 +64,11 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
  *   "col_e"   elements per thread of the column Fresnel pass (16, or 32:
  *             8 columns per workgroup exchanging through half the LDS)
+ *   "streams" S = 2..4 (default 2): the Fresnel passes of S contiguous sets
+ *             of the wavelengths run as S chains on S streams (set k started
+ *             after set k - 1's first row pass) and join before the
+ *             extraction; 1: one stream.  Frames under atsim_sim_profile use
+ *             one stream.
  *   "graph"   1: atsim_sim_frame replays a HIP graph of the frame's kernels,
  *             captured at the second frame (and again when the output
  *             pointers change); the position upload stays outside the graph

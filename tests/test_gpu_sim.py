@@ -213,3 +213,32 @@ def test_block_rule_output_not_a_multiple_of_four(ctx, oracle, Nout):
     # does not take (Nout % 4 != 0), blocks of 20 and 21 pixels
     c = _cfg1_like(oracle, N=128, Nout=Nout, M=256, unwrap_mode=[0, 1])
     _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
+
+
+@pytest.mark.parametrize("streams,col_e", [(2, 16), (2, 32), (3, 16), (4, 16)])
+def test_wavelength_sets_on_streams(oracle, streams, col_e):
+    # wavelength sets on several streams give the one-stream frames exactly
+    # (eager and graph-replayed frames) and match the oracle
+    from atmosphericturbulencesimul_amd.api import Context
+    lams = [0.5e-6, 0.8e-6, 1.25e-6, 1.65e-6, 2.2e-6]
+    c = _cfg1_like(oracle, N=128, Nout=100, M=256, lambdas=lams, unwrap_mode=[0, 1, 1, 0, 1])
+    outs = []
+    for ns in (1, streams):
+        cx = Context(0)
+        cx.set_tuning("streams", ns)
+        cx.set_tuning("col_e", col_e)
+        cx.set_tuning("graph", 1)
+        gs = gpu_sim(cx, c)
+        frames = []
+        for _ in range(4):
+            pha, amp = gs.frame()
+            frames.append((pha.cpu().numpy().copy(), amp.cpu().numpy().copy()))
+        outs.append(frames)
+        cx.close()
+    for (p1, a1), (p2, a2) in zip(*outs):
+        assert np.array_equal(p1, p2) and np.array_equal(a1, a2)
+    cx = Context(0)
+    cx.set_tuning("streams", streams)
+    cx.set_tuning("col_e", col_e)
+    _compare(c, oracle_sims(oracle, c), gpu_sim(cx, c), nframes=2)
+    cx.close()
