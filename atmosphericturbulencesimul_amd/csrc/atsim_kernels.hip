@@ -343,7 +343,12 @@ template <int N>
 struct LamCfg {
     static constexpr int T = N / 16;
     static constexpr int MW0 = 16384 / N;
-    static constexpr int MW = MW0 < 8 ? (MW0 < 1 ? 1 : MW0) : 8;
+    static constexpr int MW = MW0 < 8 ? (MW0 < 1 ? 1 : MW0) : 8;  // most wavelengths per workgroup
+    // default: at most 512 threads and N (N + N/16) 8 / 2 bytes of LDS per
+    // workgroup, so that two workgroups share a CU and one's loads and
+    // stores overlap the other's arithmetic (2048: 172 vs 202 us for 8)
+    static constexpr int MWD0 = 8192 / N;
+    static constexpr int MWD = MWD0 < MW ? (MWD0 < 1 ? 1 : MWD0) : MW;
     static constexpr int SEG = N < 2048 ? N : 2048;
 };
 
@@ -430,7 +435,7 @@ template <int N>
 static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const float2* tf, const float2* tw,
                                        const ScreenArgs& sa, int mw, hipStream_t st) {
     constexpr int MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG;
-    int mwc = (mw > 0 && mw < MW) ? mw : MW;
+    int mwc = (mw > 0 && mw < MW) ? mw : (mw >= MW ? MW : LamCfg<N>::MWD);
     const int nwl = sa.wcnt > 0 ? sa.wcnt : sa.W;  // wavelengths of this launch
     if (nwl < mwc) mwc = nwl;
     size_t lds = (size_t)mwc * Plan<N>::LDS * sizeof(float2);

@@ -60,7 +60,8 @@ int atsim_copy_async(atsim_ctx *ctx, int slot, void *dst, const void *src, size_
 int atsim_copy_wait(atsim_ctx *ctx, int slot);
 int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
 /* Launch-shape tuning of this context (0 restores the built-in default):
- *   "row_mw"  wavelength fields per row-pass workgroup (1..8)
+ *   "row_mw"  wavelength fields per row-pass workgroup (1..8; default
+ *             8192 / n, at most 8: two workgroups per CU)
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
  *   "col_e"   elements per thread of the column Fresnel pass (16, or 32:
  *             8 columns per workgroup exchanging through half the LDS)
