@@ -41,6 +41,16 @@ struct atsim_ctx {
     size_t nn_bias_cap = 0;
 };
 
+// A copy ordered on the context stream and complete on return.  hipMemcpy
+// alone is not ordered with the library's non-blocking streams, and a
+// device-to-device (or pageable host-to-device) hipMemcpy may return before
+// the data has landed, so kernels enqueued next could read it half-copied.
+static hipError_t copy_on_stream(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, nbytes, hipMemcpyDefault, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
 static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -70,7 +80,7 @@ static int get_twiddles(atsim_ctx* c, long n, float2** out) {
     }
     float2* d = nullptr;
     HIPCHK(c, hipMalloc(&d, sizeof(float2) * n));
-    HIPCHK(c, hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, copy_on_stream(c, d, h.data(), sizeof(float2) * n));
     c->twiddles[n] = d;
     *out = d;
     return 0;
@@ -158,8 +168,7 @@ int atsim_free(atsim_ctx* c, void* ptr) {
 
 int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
     if (!c || !dst || !src) return ATSIM_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(dst, src, nbytes, hipMemcpyDefault));
+    HIPCHK(c, copy_on_stream(c, dst, src, nbytes));
     return ATSIM_OK;
 }
 
@@ -711,18 +720,18 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         for (long g = 1; g <= G; g++)
             for (long w = 0; w < s->W; w++)
                 fresnel_table(h.data() + ((g - 1) * s->W + w) * s->N, s->N, s->dx, s->groups[g].dz, s->lambda[w]);
-        if (hipMemcpy(s->tf, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (copy_on_stream(c, s->tf, h.data(), sizeof(float2) * h.size()) != hipSuccess)
             r = fail(c, ATSIM_EHIP, "sim: table upload failed");
     }
     // block of output column / row i: (long)(1.0 * i / Nout * xs) (AT.c:3611)
     std::vector<int> blk(s->Nout > 0 ? s->Nout : 1, 0);
     for (long i = 0; i < s->Nout; i++) blk[i] = (int)((double)i / s->Nout * s->xs);
     if (r == ATSIM_OK) {
-        if (hipMemcpy(s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(s->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        if (copy_on_stream(c, s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size()) != hipSuccess ||
+            copy_on_stream(c, s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W) != hipSuccess ||
+            copy_on_stream(c, s->blk, blk.data(), sizeof(int) * blk.size()) != hipSuccess ||
             (s->nblockfields > 0 &&
-             hipMemcpy(s->flist_dev, flist.data(), sizeof(int) * flist.size(), hipMemcpyHostToDevice) != hipSuccess))
+             copy_on_stream(c, s->flist_dev, flist.data(), sizeof(int) * flist.size()) != hipSuccess))
             r = fail(c, ATSIM_EHIP, "sim: parameter upload failed");
     }
     if (r == ATSIM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, ATSIM_EHIP, "sim: init sync failed");
@@ -744,8 +753,7 @@ float* atsim_sim_master(atsim_sim* s, long layer) {
 int atsim_sim_set_master(atsim_sim* s, long layer, const float* src) {
     if (!s || !src || layer < 0 || layer >= s->L) return ATSIM_EINVAL;
     atsim_ctx* c = s->ctx;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(s->masters + (size_t)layer * s->M * s->M, src, sizeof(float) * s->M * s->M, hipMemcpyDefault));
+    HIPCHK(c, copy_on_stream(c, s->masters + (size_t)layer * s->M * s->M, src, sizeof(float) * s->M * s->M));
     return ATSIM_OK;
 }
 
@@ -1037,8 +1045,7 @@ int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
         memset(out, 0, sizeof(double) * need);
         return ATSIM_OK;
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, s->bsum, sizeof(double) * need, hipMemcpyDeviceToHost));
+    HIPCHK(c, copy_on_stream(c, out, s->bsum, sizeof(double) * need));
     return ATSIM_OK;
 }
 

@@ -12,6 +12,10 @@
   (phase screens and the Fresnel transfer function are unitary), and without
   propagation the unwrapped phase equals the FP64 sum of the sampled screens
   from the oracle's sampler at 512 random pixels.
+* 512^2 / 1024^2 / 2048^2 over one set of 8192^2 masters, 10 layers,
+  8 wavelengths and 16 directions on a 4 x 4 grid at -1, -1/3, 1/3, 1 arcmin:
+  energy of all 128 fields at every size, FP64 positions, and at 512^2 the
+  two corner directions against the oracle.
 
 Bars as in test_gpu_sim: relative L2 1e-5 for fields and amplitudes; phase
 relative L2 1e-5 on agreeing 2 pi decisions, at most 0.1 % disagreeing."""
@@ -119,3 +123,80 @@ def test_headline_energy_and_sampled_phase(ctx, oracle):
     assert rel_l2(g, phi) < 1e-6
     sim.close()
     sim2.close()
+
+
+def test_mixed_sizes_sixteen_directions(ctx, oracle):
+    # 512^2, 1024^2 and 2048^2 simulations (one per grid size) over the same
+    # 10 masters of 8192^2, 8 wavelengths and 16 directions on a 4 x 4 grid
+    # at -1, -1/3, 1/3, 1 arcmin (offset angle x altitude / dx per layer):
+    # energy of all 128 fields at every size, FP64 positions of every
+    # direction, and at 512^2 two corner directions against the oracle
+    from atmosphericturbulencesimul_amd.api import Simulation, hip_copy, superlayers
+    print("case: N in 512/1024/2048, M=8192, 10 layers, 8 wavelengths, 16 directions", flush=True)
+    L, M, dx, dt = 10, 8192, 0.01, 0.001
+    alts = 20000.0 * np.arange(L) / 9
+    cn2 = np.exp(-alts / 2000.0)
+    cn2 /= cn2.sum()
+    speeds = np.linspace(5.0, 40.0, L)
+    dirs = np.random.default_rng(0xA75 + 3000).uniform(0.0, 2 * math.pi, L)
+    lams = np.linspace(0.5e-6, 2.2e-6, 8)
+    ang = np.array([-1.0, -1.0 / 3, 1.0 / 3, 1.0]) * math.pi / (180 * 60)
+    dgrid = [(gx, gy) for gy in ang for gx in ang]
+    D = len(dgrid)
+    x0 = np.array([[0.05 * M + gx * alts[l] / dx for l in range(L)] for gx, _ in dgrid])
+    y0 = np.array([[0.05 * M + gy * alts[l] / dx for l in range(L)] for _, gy in dgrid])
+    vx = speeds * np.cos(dirs) / dx
+    vy = speeds * np.sin(dirs) / dx
+    sa, sli = superlayers(alts, cn2, 1000.0)
+    scoeff = np.array([[0.5e-6 / lam] * L for lam in lams])
+    unwrap = [0] + [1] * 7
+    sims = {}
+    for N in (512, 1024, 2048):
+        s = Simulation(ctx, N, N, M, sa, sli, lams, scoeff, unwrap, x0, y0, vx, vy, dx, dt, 0.0, True)
+        if not sims:
+            s.synth_masters(0xA75 + 3000, 20.0 / dx, 0.01 / dx)
+            s.normalize_masters(0.12 / dx, cn2)
+        else:
+            first = next(iter(sims.values()))
+            for l in range(L):
+                hip_copy(ctx, s.master_ptr(l), first.master_ptr(l), 4 * M * M)
+        sims[N] = s
+    for N, s in sims.items():
+        for k in range(2):
+            pha, amp = s.frame()
+            e = (amp.double() ** 2).sum(dim=(-1, -2)).cpu().numpy().ravel() / (N * N)
+            print(f"N={N} frame {k}: energy / N^2 - 1 =", np.abs(e - 1).max(), flush=True)
+            assert e.size == D * len(lams) and np.abs(e - 1).max() < 1e-5
+        xg, yg = s.positions()
+        # two frames of x += v dt from the offsets, FP64 on the host (no wrap this close to the start)
+        xe, ye = x0.copy(), y0.copy()
+        for _ in range(2):
+            xe = xe + vx * dt
+            ye = ye + vy * dt
+        xe = np.where(xe < 0, xe + M, xe)
+        ye = np.where(ye < 0, ye + M, ye)
+        assert np.allclose(xg, xe, rtol=0, atol=1e-9) and np.allclose(yg, ye, rtol=0, atol=1e-9)
+    # 512^2 against the oracle for the corner directions 0 and 15, all wavelengths
+    s = sims[512]
+    masters = np.stack([s.master(l).cpu().numpy() for l in range(L)]).astype(np.float64)
+    osims = {d: oracle.Sim(512, 512, masters, sa, sli, lams, scoeff, unwrap, x0[d], y0[d], vx, vy, dx, dt, 0.0, True)
+             for d in (0, D - 1)}
+    field = torch.empty((D, len(lams), 512, 512), dtype=torch.complex64, device="cuda")
+    s2 = Simulation(ctx, 512, 512, M, sa, sli, lams, scoeff, unwrap, x0, y0, vx, vy, dx, dt, 0.0, True)
+    for l in range(L):
+        hip_copy(ctx, s2.master_ptr(l), s.master_ptr(l), 4 * M * M)
+    for k in range(2):
+        pha_g, amp_g = s2.frame(field)
+        fg, pg, ag = field.cpu().numpy(), pha_g.cpu().numpy(), amp_g.cpu().numpy()
+        for d, os_ in osims.items():
+            E_o, _, pha_o, amp_o = os_.frame()
+            for w in range(len(lams)):
+                assert rel_l2(fg[d, w], E_o[w]) < 1e-5
+                assert rel_l2(ag[d, w], amp_o[w]) < 1e-5
+                bad = np.abs(pg[d, w] - pha_o[w]) > 1.0
+                assert bad.sum() <= 1e-3 * 512 * 512
+                assert rel_l2(pg[d, w][~bad], pha_o[w][~bad]) < 1e-5
+        print(f"oracle frame {k} ok", flush=True)
+    del masters
+    for x in list(sims.values()) + [s2]:
+        x.close()
