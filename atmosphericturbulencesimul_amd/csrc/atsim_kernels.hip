@@ -89,6 +89,32 @@ __device__ __forceinline__ float2 expi_d(double ph) {
     return make_float2(cr, sr);
 }
 
+// e^{i (pi/2) qt} for an FP64 phase in quarter turns (the row pass keeps its
+// sampled screen pre-scaled by 2/pi, so a wavelength's phase costs one
+// product): qt = n + t with n = rint(qt), |t| <= 1/2; sin and cos of
+// (pi/2) t are degree-7 / degree-8 polynomials in t (least-squares fits of
+// the absolute error on |t| <= 1/2, written here; 1.1e-7 and 8.6e-8 in FP32
+// Horner form), then rotated by n quarter turns.
+__device__ __forceinline__ float2 expi_q(double qt) {
+    const double q4 = rint(qt);
+    const float t = (float)(qt - q4);
+    const int qi = (int)q4 & 3;  // quarter turns mod 4 (two's complement)
+    const float t2 = t * t;
+    float s = fmaf(t2, -0.004592288751155138f, 0.07967590540647507f);
+    s = fmaf(t2, s, -0.6459629535675049f);
+    s = fmaf(t2, s, 1.570796251296997f);
+    s *= t;
+    float c = fmaf(t2, 0.0009036321425810456f, -0.02086007222533226f);
+    c = fmaf(t2, c, 0.25366920232772827f);
+    c = fmaf(t2, c, -1.2337005138397217f);
+    c = fmaf(t2, c, 1.0f);
+    float sr = (qi & 1) ? c : s;
+    float cr = (qi & 1) ? s : c;
+    if (qi & 2) sr = -sr;
+    if ((qi + 1) & 2) cr = -cr;
+    return make_float2(cr, sr);
+}
+
 __device__ __forceinline__ double screen_phase(const ScreenArgs& sa, int d, int w, int i, int j) {
     double ph = 0.0;
     for (int q = 0; q < sa.lcount; q++) {
@@ -409,13 +435,13 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
                     h = fma(p.wx[3], (double)r[3], h);
                     v = fma(p.wy[b], h, v);
                 }
-                V[i] = v;
+                V[i] = v * 0.63661977236758134308;  // in quarter turns (2 / pi)
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int idx = reg_index<N, RI>(t, q) - seg0;
-                if (SEG == N || (idx >= 0 && idx < SEG)) v[q] = cmul(v[q], expi_d(sw * V[idx]));
+                if (SEG == N || (idx >= 0 && idx < SEG)) v[q] = cmul(v[q], expi_q(sw * V[idx]));
             }
         }
     }
