@@ -882,7 +882,7 @@ __device__ __forceinline__ float wave_inclusive_sum(float x) {
 // by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l and the
 // screen of the last layer group, applied here after the last Fresnel gap.
 constexpr int EXT_T = 256;
-constexpr int EXT_LC = 10;  // layers staged per LDS fill
+constexpr int EXT_LC = 5;   // layers staged per LDS fill (33 KB of LDS: 4 workgroups per CU)
 template <bool FULL>
 __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     __shared__ float S[EXT_LC][4][EXT_T + 4];
