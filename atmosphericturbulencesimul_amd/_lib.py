@@ -76,6 +76,7 @@ _SIGS = {
     "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_sim_block_sums": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long]),
+    "atsim_block_fixup": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long, C.POINTER(C.c_double)]),
     "atsim_host_alloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t]),
     "atsim_host_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_copy_async": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),

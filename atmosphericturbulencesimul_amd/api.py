@@ -79,6 +79,16 @@ class Context:
         torch.cuda.current_stream(t.device).synchronize()
         return t
 
+    def block_fixup(self, bsum):
+        """(xs, xs) block peaks after the fix-up heuristic, from (xs, xs, 2) block sums"""
+        b = np.ascontiguousarray(bsum, np.float64)
+        xs = b.shape[0]
+        assert b.shape == (xs, xs, 2)
+        out = np.zeros((xs, xs))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self._chk(self.L.atsim_block_fixup(self.h, dp(b), xs, dp(out)))
+        return out
+
     def set_tuning(self, key, value):
         self._chk(self.L.atsim_set_tuning(self.h, key.encode(), int(value)))
 

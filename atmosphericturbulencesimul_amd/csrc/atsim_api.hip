@@ -528,6 +528,7 @@ struct atsim_sim {
     double* bsum = nullptr;
     double* bin = nullptr;
     double* ch = nullptr;
+    unsigned* cmax = nullptr;  // [F] largest first-round count of each block-rule field
     int* flist_dev = nullptr;
     int nblockfields = 0;
     int xs = 0;
@@ -613,6 +614,7 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->bsum);
     hipFree(s->bin);
     hipFree(s->ch);
+    hipFree(s->cmax);
     hipFree(s->flist_dev);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
@@ -709,6 +711,7 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         SIMALLOC(s->bsum, sizeof(double) * F * nb * 2);
         SIMALLOC(s->bin, sizeof(double) * F * nb);
         SIMALLOC(s->ch, sizeof(double) * F * nb);
+        SIMALLOC(s->cmax, sizeof(unsigned) * F);
         SIMALLOC(s->flist_dev, sizeof(int) * flist.size());
     }
 #undef SIMALLOC
@@ -915,7 +918,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     prof_end(s, pe);
     if (s->nblockfields > 0) {
         long pb = prof_begin(s, PROF_BLOCK);
-        HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->flist_dev, s->nblockfields, s->xs, c->stream));
+        HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->cmax, s->flist_dev, s->nblockfields, s->xs, c->stream));
         HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields, (int)s->Nout,
                                      s->xs, c->stream));
         prof_end(s, pb);
@@ -1047,6 +1050,34 @@ int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
     }
     HIPCHK(c, copy_on_stream(c, out, s->bsum, sizeof(double) * need));
     return ATSIM_OK;
+}
+
+int atsim_block_fixup(atsim_ctx* c, const double* bsum, long xs, double* peaks) {
+    if (!c || !bsum || !peaks || xs <= 0 || xs > 4096) return ATSIM_EINVAL;
+    const size_t nb = (size_t)xs * xs;
+    double *d_sum = nullptr, *d_bin = nullptr, *d_ch = nullptr;
+    unsigned* d_max = nullptr;
+    int* d_f = nullptr;
+    int r = ATSIM_OK;
+    if (hipMalloc(&d_sum, sizeof(double) * nb * 2) != hipSuccess || hipMalloc(&d_bin, sizeof(double) * nb) != hipSuccess ||
+        hipMalloc(&d_ch, sizeof(double) * nb) != hipSuccess || hipMalloc(&d_max, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&d_f, sizeof(int)) != hipSuccess)
+        r = fail(c, ATSIM_ENOMEM, "block fix-up: out of device memory");
+    const int f0 = 0;
+    if (r == ATSIM_OK && (copy_on_stream(c, d_sum, bsum, sizeof(double) * nb * 2) != hipSuccess ||
+                          copy_on_stream(c, d_f, &f0, sizeof(int)) != hipSuccess))
+        r = fail(c, ATSIM_EHIP, "block fix-up: upload failed");
+    if (r == ATSIM_OK && launch_block_fix(d_sum, d_bin, d_ch, d_max, d_f, 1, (int)xs, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "block fix-up: launch failed");
+    if (r == ATSIM_OK && copy_on_stream(c, peaks, d_bin, sizeof(double) * nb) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "block fix-up: download failed");
+    (void)hipStreamSynchronize(c->stream);
+    hipFree(d_sum);
+    hipFree(d_bin);
+    hipFree(d_ch);
+    hipFree(d_max);
+    hipFree(d_f);
+    return r;
 }
 
 int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {

@@ -1107,55 +1107,108 @@ __constant__ int c_pairs[12][6] = {
     {0, 0, 2, 0, 2, 0}, {0, 2, 0, 0, 0, 2}, {1, 2, 0, 0, 1, 2}, {0, 2, 1, 0, 1, 2},
     {2, 2, 0, 0, 2, 2}, {0, 2, 2, 0, 2, 2}, {0, 1, 2, 0, 2, 1}, {2, 1, 0, 0, 2, 1}};
 
-__global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ bsum, double* __restrict__ bin,
-                                                    double* __restrict__ ch, const int* __restrict__ flist, int xs,
-                                                    int use_lds) {
+// the fix-up count of block b: +-0.2 / +-0.15 for every neighbour relation
+// whose peaks differ by more than pi (AT.c:3667-3903)
+__device__ __forceinline__ double block_count(const double* B, int b, int xs) {
+    const int i2 = b % xs, j2 = b / xs;
+    const double pb = B[b];
+    double c = 0.0;
+    for (int q = 0; q < 12; q++) {
+        const int dx1 = c_pairs[q][0], dy1 = c_pairs[q][1], dx2 = c_pairs[q][2], dy2 = c_pairs[q][3];
+        const int xm = c_pairs[q][4], ym = c_pairs[q][5];
+        const double cq = q < 4 ? 0.2 : 0.15;
+        int x = i2 - dx1, y = j2 - dy1;  // b as the first block of the pair
+        if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
+            const double pv2 = B[(y + dy2) * xs + x + dx2];
+            if (pv2 > pb + M_PI) c += cq;
+            if (pb > pv2 + M_PI) c -= cq;
+        }
+        x = i2 - dx2; y = j2 - dy2;  // b as the second block
+        if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
+            const double pv1 = B[(y + dy1) * xs + x + dx1];
+            if (pb > pv1 + M_PI) c -= cq;
+            if (pv1 > pb + M_PI) c += cq;
+        }
+    }
+    return c;
+}
+
+// The first round of the fix-up, spread over the whole GPU (one round is all
+// that realistic fields take): the peaks atan2 of the block sums, then each
+// block's count and, per field, the largest |count| as the bits of a
+// non-negative float (ordered like unsigned integers).
+__global__ void __launch_bounds__(256) k_block_peaks(const double* __restrict__ bsum, double* __restrict__ bin,
+                                                     const int* __restrict__ flist, int xs,
+                                                     unsigned* __restrict__ cmax) {
+    const int f = flist[blockIdx.y];
+    const int nb = xs * xs;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cmax[blockIdx.y] = 0u;
+    if (b < nb) bin[(size_t)f * nb + b] = atan2(bsum[((size_t)f * nb + b) * 2 + 1], bsum[((size_t)f * nb + b) * 2]);
+}
+
+__global__ void __launch_bounds__(256) k_block_counts(const double* __restrict__ bin, double* __restrict__ ch,
+                                                      const int* __restrict__ flist, int xs,
+                                                      unsigned* __restrict__ cmax) {
+    const int f = flist[blockIdx.y];
+    const int nb = xs * xs;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    float a = 0.f;
+    if (b < nb) {
+        const float c = (float)block_count(bin + (size_t)f * nb, b, xs);
+        reinterpret_cast<float*>(ch)[(size_t)f * nb + b] = c;
+        a = fabsf(c);
+    }
+    for (int o = 32; o > 0; o >>= 1) a = fmaxf(a, __shfl_down(a, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&cmax[blockIdx.y], __float_as_uint(a));
+}
+
+// The iterative fix-up from the first round's peaks, counts and largest
+// count (k_block_peaks, k_block_counts); further rounds on one workgroup per
+// field, the block grid in LDS.
+__global__ void __launch_bounds__(1024) k_block_fix(double* __restrict__ bin, double* __restrict__ ch,
+                                                    const int* __restrict__ flist, int xs, int use_lds,
+                                                    const unsigned* __restrict__ cmax) {
     __shared__ double redd[16];
     __shared__ int redi[16];
     extern __shared__ double dyn[];  // [nb] B (double), [nb] ch (float) when use_lds
     const int f = flist[blockIdx.x];
     const int nb = xs * xs;
-    const double* S = bsum + (size_t)f * nb * 2;
     double* Bg = bin + (size_t)f * nb;
     double* B = use_lds ? dyn : Bg;
-    float* Cc = use_lds ? reinterpret_cast<float*>(dyn + nb) : reinterpret_cast<float*>(ch) + (size_t)f * nb;
+    float* Cg = reinterpret_cast<float*>(ch) + (size_t)f * nb;
+    float* Cc = use_lds ? reinterpret_cast<float*>(dyn + nb) : Cg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) B[b] = atan2(S[2 * b + 1], S[2 * b]);
+    // no first-round count above 0.5: the first round changes no peak
+    // (plim = max(m - 0.001, 0.5)) and ends the loop, so the peaks in bin
+    // are final (uniform per workgroup)
+    if (__uint_as_float(cmax[blockIdx.x]) <= 0.5f) return;
+    if (use_lds)
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            B[b] = Bg[b];
+            Cc[b] = Cg[b];
+        }
     __syncthreads();
     int chiter = 0, chcnt0cnt = 0;
     double plim = 1.0;
     while (plim > 0.51 && chiter < 1000 && chcnt0cnt < 5) {
         chiter++;
-        double mx = 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const int i2 = b % xs, j2 = b / xs;
-            const double pb = B[b];
-            double c = 0.0;
-            for (int q = 0; q < 12; q++) {
-                const int dx1 = c_pairs[q][0], dy1 = c_pairs[q][1], dx2 = c_pairs[q][2], dy2 = c_pairs[q][3];
-                const int xm = c_pairs[q][4], ym = c_pairs[q][5];
-                const double cq = q < 4 ? 0.2 : 0.15;
-                int x = i2 - dx1, y = j2 - dy1;  // b as the first block of the pair
-                if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
-                    const double pv2 = B[(y + dy2) * xs + x + dx2];
-                    if (pv2 > pb + M_PI) c += cq;
-                    if (pb > pv2 + M_PI) c -= cq;
-                }
-                x = i2 - dx2; y = j2 - dy2;  // b as the second block
-                if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
-                    const double pv1 = B[(y + dy1) * xs + x + dx1];
-                    if (pb > pv1 + M_PI) c -= cq;
-                    if (pv1 > pb + M_PI) c += cq;
-                }
+        double m;
+        if (chiter == 1) {
+            m = (double)__uint_as_float(cmax[blockIdx.x]);  // the first round's counts are in Cc
+        } else {
+            double mx = 0.0;
+            for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+                const double c = block_count(B, b, xs);
+                Cc[b] = (float)c;
+                mx = fmax(mx, fabs((double)(float)c));
             }
-            Cc[b] = (float)c;
-            mx = fmax(mx, fabs((double)(float)c));
+            for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+            if (lane == 0) redd[wv] = mx;
+            __syncthreads();
+            m = 0.0;
+            for (int q = 0; q < nw; q++) m = fmax(m, redd[q]);
         }
-        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
-        if (lane == 0) redd[wv] = mx;
-        __syncthreads();
-        double m = 0.0;
-        for (int q = 0; q < nw; q++) m = fmax(m, redd[q]);
         plim = m - 0.001;
         if (plim < 0.5) plim = 0.5;
         int cnt = 0;
@@ -1180,17 +1233,21 @@ __global__ void __launch_bounds__(1024) k_block_fix(const double* __restrict__ b
         for (int b = threadIdx.x; b < nb; b += blockDim.x) Bg[b] = B[b];
 }
 
-hipError_t launch_block_fix(double* bsum, double* bin, double* ch, const int* field_list, int nfields, int xs,
-                            hipStream_t st) {
+hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cmax, const int* field_list,
+                            int nfields, int xs, hipStream_t st) {
     if (nfields <= 0 || xs <= 0) return hipSuccess;
-    const size_t need = (size_t)xs * xs * (sizeof(double) + sizeof(float));
+    const int nb = xs * xs;
+    dim3 grid((nb + 255) / 256, nfields);
+    k_block_peaks<<<grid, 256, 0, st>>>(bsum, bin, field_list, xs, cmax);
+    k_block_counts<<<grid, 256, 0, st>>>(bin, ch, field_list, xs, cmax);
+    const size_t need = (size_t)nb * (sizeof(double) + sizeof(float));
     const int use_lds = need <= 150 * 1024;
     static bool attr = false;
     if (!attr) {
         allow_dynamic_lds((const void*)k_block_fix);
         attr = true;
     }
-    k_block_fix<<<nfields, 1024, use_lds ? need : 0, st>>>(bsum, bin, ch, field_list, xs, use_lds);
+    k_block_fix<<<nfields, 1024, use_lds ? need : 0, st>>>(bin, ch, field_list, xs, use_lds, cmax);
     return hipGetLastError();
 }
 

@@ -213,6 +213,13 @@ int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
  * reference. */
 int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
 
+/* The block fix-up of the science-lambda rule on given block sums: bsum
+ * [xs][xs][2] (host; sum cos, sum sin per block), peaks [xs][xs] (host) the
+ * block peaks after the reference's iterative +-2 pi consistency heuristic
+ * (AtmosphericTurbulence.c:3627-4262), as atsim_sim_frame computes them.
+ * xs <= 4096.  A test entry point with no counterpart in the reference. */
+int atsim_block_fixup(atsim_ctx *ctx, const double *bsum, long xs, double *peaks);
+
 /* Advance every layer by one time step without computing a frame (the
  * positions a following atsim_sim_layer_screens samples).  No counterpart in
  * the reference: the surrogate's frames skip the layer loop. */

@@ -242,3 +242,28 @@ def test_wavelength_sets_on_streams(oracle, streams, col_e):
     cx.set_tuning("col_e", col_e)
     _compare(c, oracle_sims(oracle, c), gpu_sim(cx, c), nframes=2)
     cx.close()
+
+
+@pytest.mark.parametrize("case", ["identity", "isolated_block", "branch_cut"])
+def test_block_fixup_against_the_oracle(ctx, oracle, case):
+    # the GPU fix-up heuristic on block sums from planted fields: nothing to
+    # fix, an isolated block flipped in 2 rounds, and a straight branch cut
+    # that runs into the 1000-round cap (AtmosphericTurbulence.c:3627-4262)
+    n = 200
+    y, x = np.mgrid[0:n, 0:n]
+    if case == "identity":
+        PHI = 25.0 * np.sin(x / 9.0) + 0.2 * y
+        off = np.zeros((n, n))
+    elif case == "isolated_block":
+        PHI = 12.0 * np.cos(y / 30.0) + 0.05 * x
+        off = np.full((n, n), 0.9 * np.pi)
+        off[100:120, 100:120] = 1.1 * np.pi
+    else:
+        PHI = 12.0 * np.cos(y / 30.0)
+        off = 0.6 * np.pi + 0.8 * np.pi * x / n
+    E = np.exp(1j * (PHI + off))
+    _, _, peak_o, it = oracle.extract_block(E, PHI, n)
+    bre, bim = _block_sums_np(E, PHI, n)
+    peak_g = ctx.block_fixup(np.stack([bre, bim], -1))
+    print(case, "oracle rounds", it, "max |peak diff|", np.abs(peak_g - peak_o).max())
+    assert np.allclose(peak_g, peak_o, rtol=0, atol=1e-9)
