@@ -339,7 +339,9 @@ class Simulation:
         hip_copy(self.ctx, out.data_ptr(), self.master_ptr(layer), 4 * self.M * self.M)
         return out
 
-    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3}
+    # row / col: passes with one Fresnel factor; row2 / col2: passes with a
+    # factor, a layer group's screen and a factor (one pass per gap)
+    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3, "row2": 4, "col2": 5, "phase": 6}
 
     def profile(self, enable=True):
         self.ctx._chk(self.ctx.L.atsim_sim_profile(self.h, int(enable)))

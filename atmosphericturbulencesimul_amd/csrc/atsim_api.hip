@@ -31,6 +31,8 @@ struct atsim_ctx {
     int col_c = 0;   // columns per column-pass workgroup
     int col_e = 0;   // elements per thread of the column Fresnel pass (0 = default, 16, 32)
     int streams = 2; // 2..4: the Fresnel passes of that many wavelength sets run on as many streams (1: one)
+    int gap_passes = 1;  // 1: one pass over the field per gap (x / y order alternating), 2: a row and a column pass
+    int col2_e = 0;      // elements per thread of the column passes with a screen (0 = default, 16, 32)
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
@@ -237,6 +239,12 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else if (!strcmp(key, "streams")) {
         if (value < 0 || value > 4) return fail(c, ATSIM_EINVAL, "streams must be in [0, 4]");
         c->streams = value == 0 ? 2 : value;
+    } else if (!strcmp(key, "col2_e")) {
+        if (value != 0 && value != 16 && value != 32) return fail(c, ATSIM_EINVAL, "col2_e must be 0, 16 or 32");
+        c->col2_e = value;
+    } else if (!strcmp(key, "gap_passes")) {
+        if (value < 0 || value > 2) return fail(c, ATSIM_EINVAL, "gap_passes must be 0, 1 or 2");
+        c->gap_passes = value == 0 ? 1 : value;
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "graph must be 0 or 1");
         c->graph = value;
@@ -252,6 +260,8 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!strcmp(key, "col_c")) return c->col_c;
     if (!strcmp(key, "col_e")) return c->col_e;
     if (!strcmp(key, "streams")) return c->streams;
+    if (!strcmp(key, "gap_passes")) return c->gap_passes;
+    if (!strcmp(key, "col2_e")) return c->col2_e;
     if (!strcmp(key, "graph")) return c->graph;
     return ATSIM_EINVAL;
 }
@@ -497,6 +507,7 @@ long atsim_superlayers(const double* alt, const double* cn2, long L, double bin,
 struct Group {
     int lfirst, lcount;  // layers [lfirst, lfirst + lcount)
     double dz;           // propagation distance before this group (0 for the top group)
+    int slot0 = 0;       // first phase plane of its layers (groups applied by column passes)
 };
 
 struct atsim_sim {
@@ -530,6 +541,11 @@ struct atsim_sim {
     double* ch = nullptr;
     unsigned* cmax = nullptr;  // [F] largest first-round count of each block-rule field
     int* flist_dev = nullptr;
+    // sampled phase planes of the layers whose screens column passes apply
+    // (one pass per gap): [nslots][D][N][N] FP64 quarter turns
+    double* vq = nullptr;
+    int* slot_layer_dev = nullptr;
+    int nslots = 0;
     int nblockfields = 0;
     int xs = 0;
     // per-launch event timing (bench instrumentation)
@@ -550,7 +566,16 @@ struct atsim_sim {
     } gslot[2];
 };
 
-enum { PROF_ROW = 0, PROF_COL = 1, PROF_EXTRACT = 2, PROF_BLOCK = 3, PROF_KINDS = 4 };
+enum {
+    PROF_ROW = 0,       // row pass: screen + x factor (the first pass), or x factor alone (the last)
+    PROF_COL = 1,       // column pass: y factor
+    PROF_EXTRACT = 2,
+    PROF_BLOCK = 3,
+    PROF_ROW_GAP2 = 4,  // row pass: x factor, screen, x factor
+    PROF_COL_GAP2 = 5,  // column pass: y factor, screen, y factor
+    PROF_PHASE = 6,     // sampled phase planes of the column passes' layers
+    PROF_KINDS = 7
+};
 
 static hipEvent_t prof_event(atsim_sim* s) {
     if (s->ev_used == s->ev_pool.size()) {
@@ -616,6 +641,8 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->ch);
     hipFree(s->cmax);
     hipFree(s->flist_dev);
+    hipFree(s->vq);
+    hipFree(s->slot_layer_dev);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
 }
@@ -704,6 +731,19 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         for (long w = 0; w < s->W; w++)
             if (s->unwrap[w] != 0 && s->amplitude) flist.push_back((int)(dd * s->W + w));
     s->nblockfields = (int)flist.size();
+    // one pass per gap: column pass k (odd, 1 <= k < G) applies group k
+    std::vector<int> slot_layer;
+    if (s->propagate && col_screen_supported(s->N) && G >= 2) {
+        for (long k = 1; k < G; k += 2) {
+            s->groups[k].slot0 = (int)slot_layer.size();
+            for (int q = 0; q < s->groups[k].lcount; q++) slot_layer.push_back(s->groups[k].lfirst + q);
+        }
+    }
+    s->nslots = (int)slot_layer.size();
+    if (s->nslots > 0) {
+        SIMALLOC(s->vq, sizeof(double) * s->nslots * s->D * nn);
+        SIMALLOC(s->slot_layer_dev, sizeof(int) * s->nslots);
+    }
     s->xs = (int)(s->Nout / 20);
     if (s->nblockfields > 0) {
         const size_t nb = (size_t)(s->xs > 0 ? s->xs * s->xs : 1);
@@ -734,7 +774,9 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
             copy_on_stream(c, s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W) != hipSuccess ||
             copy_on_stream(c, s->blk, blk.data(), sizeof(int) * blk.size()) != hipSuccess ||
             (s->nblockfields > 0 &&
-             copy_on_stream(c, s->flist_dev, flist.data(), sizeof(int) * flist.size()) != hipSuccess))
+             copy_on_stream(c, s->flist_dev, flist.data(), sizeof(int) * flist.size()) != hipSuccess) ||
+            (s->nslots > 0 &&
+             copy_on_stream(c, s->slot_layer_dev, slot_layer.data(), sizeof(int) * slot_layer.size()) != hipSuccess))
             r = fail(c, ATSIM_EHIP, "sim: parameter upload failed");
     }
     if (r == ATSIM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = fail(c, ATSIM_EHIP, "sim: init sync failed");
@@ -829,6 +871,62 @@ int atsim_sim_advance(atsim_sim* s) {
     return advance_positions(s);
 }
 
+// Fresnel passes of a frame.  The x and y factors of a gap commute.  With
+// one pass per gap, gap g is applied x-first for even g and y-first for odd
+// g: pass k (0 .. G; rows for even k, columns for odd k) applies the factor
+// of gap k - 1 along its axis, the screen of layer group k and the factor
+// of gap k along the same axis -- pass 0 starts from the unit field, pass G
+// has no group or gap after it.  With two passes per gap, step 2g is the
+// row pass (screen of group g, x factor of gap g) and step 2g + 1 the
+// column pass (y factor).
+static bool use_gap_fusion(const atsim_sim* s) {
+    return s->ctx->gap_passes == 1 && col_screen_supported(s->N) && (long)s->groups.size() - 1 >= 2 &&
+           s->nslots > 0;
+}
+
+static int step_kind(bool fused, long step, long G) {
+    if (!fused) return step % 2 ? PROF_COL : PROF_ROW;
+    if (step == 0 || step == G) return step % 2 ? PROF_COL : PROF_ROW;
+    return step % 2 ? PROF_COL_GAP2 : PROF_ROW_GAP2;
+}
+
+// one step of the chain for the wavelength range of sa (wcnt 0: all)
+static hipError_t launch_step(atsim_sim* s, bool fused, long step, ScreenArgs sa, hipStream_t st) {
+    atsim_ctx* c = s->ctx;
+    const long G = (long)s->groups.size() - 1, W = s->W, N = s->N, D = s->D;
+    auto tab = [&](long g) { return (const float2*)(s->tf + (size_t)g * W * N); };
+    const long nfields = D * (sa.wcnt > 0 ? sa.wcnt : W);
+    auto group = [&](long g) {
+        sa.lfirst = s->groups[g].lfirst;
+        sa.lcount = s->groups[g].lcount;
+    };
+    if (!fused) {
+        const long g = step / 2;
+        group(g);
+        if (step % 2 == 0) return launch_row_fresnel(N, g == 0, s->fields, (int)D, tab(g), s->tw, sa, st, c->row_mw);
+        return launch_col(N, OP_FRESNEL, 0, s->fields, nfields, tab(g), (int)W, s->tw, st, c->col_c, c->col_e,
+                          sa.wlo, sa.wcnt);
+    }
+    const long k = step;
+    const bool rows = k % 2 == 0;
+    if (k == 0) {
+        group(0);
+        return launch_row_fresnel(N, true, s->fields, (int)D, tab(0), s->tw, sa, st, c->row_mw);
+    }
+    if (k == G) {
+        sa.lfirst = 0;
+        sa.lcount = 0;
+        if (rows) return launch_row_fresnel(N, false, s->fields, (int)D, tab(G - 1), s->tw, sa, st, c->row_mw);
+        return launch_col(N, OP_FRESNEL, 0, s->fields, nfields, tab(G - 1), (int)W, s->tw, st, c->col_c, c->col_e,
+                          sa.wlo, sa.wcnt);
+    }
+    group(k);
+    if (rows)
+        return launch_row_fresnel(N, false, s->fields, (int)D, tab(k), s->tw, sa, st, c->row_mw, tab(k - 1));
+    return launch_col_screen(N, s->fields, (int)D, tab(k - 1), tab(k), s->tw, sa, s->vq, s->groups[k].slot0, st,
+                             c->col2_e);
+}
+
 // the kernels of one frame (positions already uploaded), in stream order
 static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     atsim_ctx* c = s->ctx;
@@ -843,29 +941,31 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     const long F = D * W;
     const long G = (long)s->groups.size() - 1;
     const int nsets = (int)std::min<long>(c->streams, W);
+    const bool fused = use_gap_fusion(s);
+    const long nsteps = G <= 0 ? 0 : (fused ? G + 1 : 2 * G);
+    auto phase = [&](hipStream_t st) {
+        return launch_layer_phase(s->vq, s->masters, s->ls_dev, s->slot_layer_dev, s->nslots, s->N, M, L, D, st);
+    };
     if (s->propagate && nsets >= 2 && !s->prof) {
-        // nsets wavelength sets, each its own chain of row / column passes:
-        // set 0 on the context stream, set k on xs_stream[k - 1], started
-        // after set k - 1's first row pass, so that one set's row pass
-        // (arithmetic-bound) runs beside another's column pass
-        // (memory-bound).  All join the context stream before extraction.
+        // nsets wavelength sets, each its own chain of passes: set 0 on the
+        // context stream, set k on xs_stream[k - 1], started after set
+        // k - 1's first pass, so that one set's row pass (arithmetic-bound)
+        // runs beside another's column pass (memory-bound).  All join the
+        // context stream before extraction.  The phase planes of the column
+        // passes are sampled first, on the context stream.
+        if (fused) HIPCHK(c, phase(c->stream));
         HIPCHK(c, hipEventRecord(s->ev_fork, c->stream));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
-        for (long g = 0; g < G; g++) {
-            sa.lfirst = s->groups[g].lfirst;
-            sa.lcount = s->groups[g].lcount;
-            const float2* tab = s->tf + (size_t)g * W * s->N;
+        for (long step = 0; step < nsteps; step++) {
             for (int k = 0; k < nsets; k++) {
                 hipStream_t st = k ? s->xs_stream[k - 1] : c->stream;
                 sa.wlo = (int)(W * k / nsets);
                 sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
-                HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, st, c->row_mw));
-                if (g == 0 && k + 1 < nsets) {
+                HIPCHK(c, launch_step(s, fused, step, sa, st));
+                if (step == 0 && k + 1 < nsets) {
                     HIPCHK(c, hipEventRecord(s->ev_first[k], st));
                     HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k], s->ev_first[k], 0));
                 }
-                HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, D * sa.wcnt, tab, (int)W, s->tw, st, c->col_c,
-                                     c->col_e, sa.wlo, sa.wcnt));
             }
         }
         sa.wlo = sa.wcnt = 0;
@@ -874,17 +974,15 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
             HIPCHK(c, hipStreamWaitEvent(c->stream, s->ev_join[k - 1], 0));
         }
     } else if (s->propagate) {
-        for (long g = 0; g < G; g++) {
-            sa.lfirst = s->groups[g].lfirst;
-            sa.lcount = s->groups[g].lcount;
-            const float2* tab = s->tf + (size_t)g * W * s->N;
-            long pr = prof_begin(s, PROF_ROW);
-            HIPCHK(c, launch_row_fresnel(s->N, g == 0, s->fields, (int)D, tab, s->tw, sa, c->stream, c->row_mw));
+        if (fused) {
+            long pp = prof_begin(s, PROF_PHASE);
+            HIPCHK(c, phase(c->stream));
+            prof_end(s, pp);
+        }
+        for (long step = 0; step < nsteps; step++) {
+            long pr = prof_begin(s, step_kind(fused, step, G));
+            HIPCHK(c, launch_step(s, fused, step, sa, c->stream));
             prof_end(s, pr);
-            long pc = prof_begin(s, PROF_COL);
-            HIPCHK(c, launch_col(s->N, OP_FRESNEL, 0, s->fields, F, tab, (int)W, s->tw, c->stream, c->col_c,
-                                 c->col_e));
-            prof_end(s, pc);
         }
     }
     ExtractArgs a{};

@@ -136,6 +136,13 @@ __device__ __forceinline__ int xcd_swizzle(int b, int total) {
     const int xcd = b & 7, r = b >> 3;
     return ((((r >> 2) << 3) | xcd) << 2) | (r & 3);
 }
+// the same with chunks of K consecutive indices per XCD (K a power of two)
+template <int K>
+__device__ __forceinline__ int xcd_chunk(int b, int total) {
+    if (total % (8 * K)) return b;
+    const int xcd = b & 7, r = b >> 3;
+    return ((r / K) * 8 + xcd) * K + r % K;
+}
 
 // Raw buffer access (vector memory): a resource over [p, p + bytes), a
 // per-lane byte offset and a scalar byte offset.  Accesses past `bytes`
@@ -148,11 +155,23 @@ __device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, uint32_t v
     const u32x2 x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
     return make_float2(__uint_as_float(x.x), __uint_as_float(x.y));
 }
+__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const u32x2 x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return __hiloint2double((int)x.y, (int)x.x);
+}
 __device__ __forceinline__ void buf_store2(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     u32x2 x;
     x.x = __float_as_uint(v.x);
     x.y = __float_as_uint(v.y);
     __builtin_amdgcn_raw_buffer_store_b64(x, r, voff, soff, 0);
+}
+
+// the value of x, opaque to the optimiser: index arithmetic derived from it
+// is recomputed instead of kept live from an earlier use of x (an empty asm
+// statement with no instructions)
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
 }
 
 // most layers a frame can hold (per-frame layer parameters are staged in LDS)
@@ -360,6 +379,175 @@ k_col_split(float2* __restrict__ data, const float2* __restrict__ tf, int W, con
     for (int q = 0; q < E; q++) buf_store2(v[q], fr, lo, (reg_index<N, RO, E>(0, q) * N) * 8);
 }
 
+// Column pass around a layer group (the x and y factors of a gap commute, so
+// the gaps alternate between x-first and y-first and one pass over the field
+// applies the y factor of the gap before the group, the group's screen and
+// the y factor of the gap after it).  Workgroup: C columns of one field.  The
+// group's layer phases come sampled (FP64 quarter turns, one plane per layer
+// and direction, k_layer_phase): the phase of a pixel is sum_l s_wl V_l,
+// applied as one e^{i phase}.  The fields of one column tile are adjacent
+// launch indices on one XCD, so they read the phase planes through its L2.
+template <int N, int C>
+__global__ void __launch_bounds__(C * (N / 16))
+k_col_screen(float2* __restrict__ data, const float2* __restrict__ tfp, const float2* __restrict__ tfc,
+             const float2* __restrict__ tw, ScreenArgs sa, const double* __restrict__ vq, int slot0) {
+    using P = Plan<N>;
+    constexpr int RI = P::first(-1), RL = P::last(-1);
+    __shared__ float2 lds[C * P::LDS];
+    static_assert(P::first(-1) == P::last(+1), "the inverse plan must end in the forward plan's layout");
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int nf = gridDim.y;
+    // 16 consecutive launch indices per XCD: with 8 fields, both 64-byte
+    // halves of a 128-byte line of every field and of the phase planes are
+    // read on one XCD
+    const int lin = xcd_chunk<16>(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int ct = lin / nf, fi = lin - ct * nf;
+    const int nw = sa.wcnt > 0 ? sa.wcnt : sa.W;
+    const int d = fi / nw, w = (sa.wcnt > 0 ? sa.wlo : 0) + (fi - d * nw);
+    const int f = d * sa.W + w;
+    const int D = nf / nw;
+    const int col0 = ct * C;
+    // buffer addressing: one lane offset (row t, column col0 + c), each
+    // register slot's row as a constant scalar offset (no per-slot address
+    // registers live across the kernel)
+    const auto fr = buf_rsrc(data + (size_t)f * N * N, (uint32_t)N * N * 8);
+    const uint32_t lo = ((uint32_t)t * N + col0 + c) * 8;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = buf_load2(fr, lo, (reg_index<N, RI>(0, q) * N) * 8);
+    fft_regs<N, -1, C>(v, lds, t, c, tw);
+    {
+        const auto hr = buf_rsrc(tfp + (size_t)w * N, (uint32_t)N * 8);
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t * 8, reg_index<N, RL>(0, q) * 8));
+    }
+    fft_regs<N, +1, C>(v, lds, t, c, tw);
+    if (sa.lcount > 0) {
+        double ph[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) ph[q] = 0.0;
+        for (int k = 0; k < sa.lcount; k++) {
+            const double sw = sa.s[(size_t)w * sa.L + sa.lfirst + k];
+            const auto vr = buf_rsrc(vq + ((size_t)(slot0 + k) * D + d) * N * N, (uint32_t)N * N * 8);
+#pragma unroll
+            for (int q = 0; q < 16; q++) ph[q] = fma(sw, buf_load_f64(vr, lo, (reg_index<N, RI>(0, q) * N) * 8), ph[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], expi_q(ph[q]));
+    }
+    // the second factor's twiddle and LDS indices are recomputed, not kept
+    // live through the screen from the first factor
+    const int t2 = opaque(t), c2 = opaque(c);
+    fft_regs<N, -1, C>(v, lds, t2, c2, tw);
+    {
+        const auto hr = buf_rsrc(tfc + (size_t)w * N, (uint32_t)N * 8);
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t2 * 8, reg_index<N, RL>(0, q) * 8));
+    }
+    fft_regs<N, +1, C>(v, lds, t2, c2, tw);
+#pragma unroll
+    for (int q = 0; q < 16; q++) buf_store2(v[q], fr, lo, (reg_index<N, RI>(0, q) * N) * 8);
+}
+
+// The same column pass with E = 32 elements per thread (N / 32 threads per
+// column) exchanging through float LDS: C = 8 columns take 8 (N + N/16) 4 B
+// (68 KB at N = 2048), so two workgroups share a CU and one's barriers and
+// loads overlap the other's arithmetic.  At most 128 VGPRs (4 waves per
+// SIMD).
+template <int N, int C, int E>
+__global__ void __launch_bounds__(C * (N / E)) __attribute__((amdgpu_waves_per_eu(4)))
+k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, const float2* __restrict__ tfc,
+                   const float2* __restrict__ tw, ScreenArgs sa, const double* __restrict__ vq, int slot0) {
+    using P = Plan<N>;
+    constexpr int RI = P::first(-1), RL = P::last(-1);
+    __shared__ float lds[C * P::LDS];
+    static_assert(P::first(-1) == P::last(+1), "the inverse plan must end in the forward plan's layout");
+    const int c = threadIdx.x % C, t = threadIdx.x / C;
+    const int nf = gridDim.y;
+    const int lin = xcd_chunk<16>(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int ct = lin / nf, fi = lin - ct * nf;
+    const int nw = sa.wcnt > 0 ? sa.wcnt : sa.W;
+    const int d = fi / nw, w = (sa.wcnt > 0 ? sa.wlo : 0) + (fi - d * nw);
+    const int f = d * sa.W + w;
+    const int D = nf / nw;
+    const int col0 = ct * C;
+    const auto fr = buf_rsrc(data + (size_t)f * N * N, (uint32_t)N * N * 8);
+    const uint32_t lo = ((uint32_t)t * N + col0 + c) * 8;
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = buf_load2(fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
+    fft_regs_split<N, -1, C, E>(v, lds, t, c, tw);
+    {
+        const auto hr = buf_rsrc(tfp + (size_t)w * N, (uint32_t)N * 8);
+#pragma unroll
+        for (int q = 0; q < E; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t * 8, reg_index<N, RL, E>(0, q) * 8));
+    }
+    fft_regs_split<N, +1, C, E>(v, lds, t, c, tw);
+    // the group's phase sum_l s_wl V_l, 8 register slots at a time
+#pragma unroll
+    for (int q0 = 0; q0 < E; q0 += 8) {
+        double ph[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) ph[u] = 0.0;
+        for (int k = 0; k < sa.lcount; k++) {
+            const double sw = sa.s[(size_t)w * sa.L + sa.lfirst + k];
+            const auto vr = buf_rsrc(vq + ((size_t)(slot0 + k) * D + d) * N * N, (uint32_t)N * N * 8);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                ph[u] = fma(sw, buf_load_f64(vr, lo, (reg_index<N, RI, E>(0, q0 + u) * N) * 8), ph[u]);
+        }
+        if (sa.lcount > 0) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[q0 + u] = cmul(v[q0 + u], expi_q(ph[u]));
+        }
+    }
+    const int t2 = opaque(t), c2 = opaque(c);
+    fft_regs_split<N, -1, C, E>(v, lds, t2, c2, tw);
+    {
+        const auto hr = buf_rsrc(tfc + (size_t)w * N, (uint32_t)N * 8);
+#pragma unroll
+        for (int q = 0; q < E; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t2 * 8, reg_index<N, RL, E>(0, q) * 8));
+    }
+    fft_regs_split<N, +1, C, E>(v, lds, t2, c2, tw);
+#pragma unroll
+    for (int q = 0; q < E; q++) buf_store2(v[q], fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
+}
+
+// Sampled phase of layer slot_layer[slot] for direction d, row j, in FP64
+// quarter turns (x 2 / pi): vq[((slot D + d) N + j) N + i].  The 4 master rows
+// of the row are staged in LDS and every pixel sums its 4 x 4 taps in the
+// order of the row pass (x taps per master row, then the rows).
+__global__ void __launch_bounds__(256) k_layer_phase(double* __restrict__ vq, const float* __restrict__ masters,
+                                                     const LayerSample* __restrict__ ls,
+                                                     const int* __restrict__ slot_layer, int N, int M, int L) {
+    extern __shared__ float2 lds[];
+    float* S = reinterpret_cast<float*>(lds);  // [4][N + 4]
+    // 4 adjacent rows (which stage overlapping master rows) on one XCD
+    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, slot = blockIdx.z, D = gridDim.y;
+    const int SW = N + 4;
+    const int l = slot_layer[slot];
+    const LayerSample* p = ls + (size_t)d * L + l;
+    const float* m = masters + (size_t)l * M * M;
+    const int ys = mod_m(p->y0 + j - 1, M), xs = mod_m(p->x0 - 1, M);
+    stage_rows4<8>(S, SW, m, M, ys, xs, N, threadIdx.x, blockDim.x);
+    stage_tail(S, SW, m, 0, M, &ys, &xs, 1, N, 3, threadIdx.x, blockDim.x);
+    __syncthreads();
+    double* out = vq + (((size_t)slot * D + d) * N + j) * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double v = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const float* r = S + b * SW + i;
+            double h = p->wx[0] * (double)r[0];
+            h = fma(p->wx[1], (double)r[1], h);
+            h = fma(p->wx[2], (double)r[2], h);
+            h = fma(p->wx[3], (double)r[3], h);
+            v = fma(p->wy[b], h, v);
+        }
+        out[i] = v * 0.63661977236758134308;
+    }
+}
+
 // Fresnel x pass for up to MW wavelength fields of the same direction and
 // spatial row j: the screen phase of the layer group is sampled once per
 // pixel (FP64, master rows staged in LDS) and shared by the wavelengths,
@@ -378,10 +566,14 @@ struct LamCfg {
     static constexpr int SEG = N < 2048 ? N : 2048;
 };
 
-template <int N, bool GEN>
+// PRE: the row first gets the x factor tfp of the gap before the group
+// (forward transform, multiply, inverse transform; the inverse leaves the
+// registers in the layout the forward transform reads), then the screen and
+// the x factor tf of the gap after it.
+template <int N, bool GEN, bool PRE>
 __global__ void __launch_bounds__(LamCfg<N>::MW * (N / 16))
 k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const float2* __restrict__ tw,
-              ScreenArgs sa) {
+              ScreenArgs sa, const float2* __restrict__ tfp) {
     using P = Plan<N>;
     constexpr int T = P::T, MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG, SW = SEG + 4;
     constexpr int RI = P::first(-1), RO = P::last(+1), RL = P::last(-1);
@@ -403,6 +595,15 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     // the field is needed only after the screen: issue its loads first
 #pragma unroll
     for (int q = 0; q < 16; q++) v[q] = (GEN || !active) ? make_float2(1.f, 0.f) : rowp[reg_index<N, RI>(t, q)];
+    static_assert(RI == RO, "the inverse plan must end in the layout the forward plan starts from");
+    if constexpr (PRE) {
+        float2* Lp = lds + lw * P::LDS;
+        fft_regs<N, -1, 1>(v, Lp, t, 0, tw);
+        const float2* hp = tfp + (size_t)w * N;
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = cmul(v[q], hp[reg_index<N, RL>(t, q)]);
+        fft_regs<N, +1, 1>(v, Lp, t, 0, tw);
+    }
     __shared__ LayerSample sls[MAX_LAYERS];
     __shared__ double ss[8][MAX_LAYERS];
     for (int k = threadIdx.x; k < sa.lcount; k += nthr) sls[k] = sa.ls[d * sa.L + sa.lfirst + k];
@@ -446,11 +647,12 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
         }
     }
     float2* Lr = lds + lw * P::LDS;
-    fft_regs<N, -1, 1>(v, Lr, t, 0, tw);
+    const int t2 = PRE ? opaque(t) : t;  // with PRE: indices recomputed, not kept live through the screen
+    fft_regs<N, -1, 1>(v, Lr, t2, 0, tw);
     const float2* h = tf + (size_t)w * N;
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t, q)]);
-    fft_regs<N, +1, 1>(v, Lr, t, 0, tw);
+    for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t2, q)]);
+    fft_regs<N, +1, 1>(v, Lr, t2, 0, tw);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 16; q++) rowp[reg_index<N, RO>(t, q)] = v[q];
@@ -459,7 +661,7 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
 
 template <int N>
 static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                                       const ScreenArgs& sa, int mw, hipStream_t st) {
+                                       const ScreenArgs& sa, int mw, hipStream_t st, const float2* tfp) {
     constexpr int MW = LamCfg<N>::MW, SEG = LamCfg<N>::SEG;
     int mwc = (mw > 0 && mw < MW) ? mw : (mw >= MW ? MW : LamCfg<N>::MWD);
     const int nwl = sa.wcnt > 0 ? sa.wcnt : sa.W;  // wavelengths of this launch
@@ -470,12 +672,15 @@ static hipError_t row_fresnel_dispatch(bool gen, float2* data, int D, const floa
     dim3 grid(N, D, (nwl + mwc - 1) / mwc), block(mwc * (N / 16));
     static bool attr = false;
     if (!attr) {
-        allow_dynamic_lds((const void*)k_row_fresnel<N, true>);
-        allow_dynamic_lds((const void*)k_row_fresnel<N, false>);
+        allow_dynamic_lds((const void*)k_row_fresnel<N, true, false>);
+        allow_dynamic_lds((const void*)k_row_fresnel<N, false, false>);
+        allow_dynamic_lds((const void*)k_row_fresnel<N, false, true>);
         attr = true;
     }
-    if (gen) k_row_fresnel<N, true><<<grid, block, lds, st>>>(data, tf, tw, sa);
-    else k_row_fresnel<N, false><<<grid, block, lds, st>>>(data, tf, tw, sa);
+    if (gen && tfp) return hipErrorInvalidValue;
+    if (gen) k_row_fresnel<N, true, false><<<grid, block, lds, st>>>(data, tf, tw, sa, nullptr);
+    else if (tfp) k_row_fresnel<N, false, true><<<grid, block, lds, st>>>(data, tf, tw, sa, tfp);
+    else k_row_fresnel<N, false, false><<<grid, block, lds, st>>>(data, tf, tw, sa, nullptr);
     return hipGetLastError();
 }
 
@@ -559,10 +764,10 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 }
 
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                              const ScreenArgs& sa, hipStream_t st, int mw) {
+                              const ScreenArgs& sa, hipStream_t st, int mw, const float2* tfp) {
     switch (n) {
 #define X(NN) \
-    case NN: return row_fresnel_dispatch<NN>(gen, data, D, tf, tw, sa, mw, st);
+    case NN: return row_fresnel_dispatch<NN>(gen, data, D, tf, tw, sa, mw, st, tfp);
         ATSIM_FOR_EACH_N(X)
 #undef X
         default: return hipErrorInvalidValue;
@@ -575,6 +780,57 @@ hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float
 #define X(NN) \
     case NN: return col_dispatch<NN>(op, dir, data, F, tf, W > 0 ? W : 1, tw, c, e, wlo, wcnt, st);
         ATSIM_FOR_EACH_N(X)
+#undef X
+        default: return hipErrorInvalidValue;
+    }
+}
+
+bool col_screen_supported(long n) { return supported_n(n) && n <= 4096; }
+
+hipError_t launch_layer_phase(double* vq, const float* masters, const LayerSample* ls, const int* slot_layer,
+                              int nslots, long N, long M, long L, long D, hipStream_t st) {
+    if (nslots <= 0) return hipSuccess;
+    static bool attr = false;
+    if (!attr) {
+        allow_dynamic_lds((const void*)k_layer_phase);
+        attr = true;
+    }
+    dim3 grid((unsigned)N, (unsigned)D, (unsigned)nslots);
+    k_layer_phase<<<grid, 256, (size_t)4 * (N + 4) * sizeof(float), st>>>(vq, masters, ls, slot_layer, (int)N,
+                                                                           (int)M, (int)L);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t col_screen_split(float2* data, int D, const float2* tfp, const float2* tf, const float2* tw,
+                                   const ScreenArgs& sa, const double* vq, int slot0, hipStream_t st) {
+    const int nw = sa.wcnt > 0 ? sa.wcnt : sa.W;
+    dim3 grid(N / 8, (unsigned)(D * nw)), block(8 * (N / 32));
+    k_col_screen_split<N, 8, 32><<<grid, block, 0, st>>>(data, tfp, tf, tw, sa, vq, slot0);
+    return hipGetLastError();
+}
+
+hipError_t launch_col_screen(long n, float2* data, int D, const float2* tfp, const float2* tf, const float2* tw,
+                             const ScreenArgs& sa, const double* vq, int slot0, hipStream_t st, int e) {
+    if (!tfp || !tf || D <= 0 || (sa.lcount > 0 && !vq)) return hipErrorInvalidValue;
+    const int nw = sa.wcnt > 0 ? sa.wcnt : sa.W;
+    if (e != 16 && n >= 256 && n <= 2048) {
+        switch (n) {
+            case 256: return col_screen_split<256>(data, D, tfp, tf, tw, sa, vq, slot0, st);
+            case 512: return col_screen_split<512>(data, D, tfp, tf, tw, sa, vq, slot0, st);
+            case 1024: return col_screen_split<1024>(data, D, tfp, tf, tw, sa, vq, slot0, st);
+            default: return col_screen_split<2048>(data, D, tfp, tf, tw, sa, vq, slot0, st);
+        }
+    }
+    switch (n) {
+#define X(NN)                                                                                         \
+    case NN: {                                                                                        \
+        constexpr int C = ColCfg<NN>::C;                                                              \
+        dim3 grid(NN / C, (unsigned)(D * nw)), block(C * (NN / 16));                                  \
+        k_col_screen<NN, C><<<grid, block, 0, st>>>(data, tfp, tf, tw, sa, vq, slot0);                \
+        return hipGetLastError();                                                                     \
+    }
+        X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 #undef X
         default: return hipErrorInvalidValue;
     }

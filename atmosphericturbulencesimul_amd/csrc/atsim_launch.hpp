@@ -20,9 +20,24 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 // Fresnel x pass of every (direction, wavelength) field with the screen
 // phase of the layer group sa.lfirst .. + sa.lcount (sampled once per pixel
 // and direction); gen: the incoming field is 1.  Tables tf + w n.
-// mw: wavelengths per workgroup (0 = largest that fits)
+// mw: wavelengths per workgroup (0 = largest that fits).  tfp (not with
+// gen): the x factor of the gap before the group is applied first (tables
+// tfp + w n).
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
-                              const ScreenArgs& sa, hipStream_t st, int mw = 0);
+                              const ScreenArgs& sa, hipStream_t st, int mw = 0, const float2* tfp = nullptr);
+// Column pass around a layer group: y factor tfp of the gap before the
+// group, the group's screen phase (layers sa.lfirst .. + sa.lcount, their
+// sampled phases in planes slot0 .. of vq), y factor tf of the gap after it;
+// every field of the wavelength range of sa (tables + w n).  n <= 4096.
+// e: elements per thread (0 = default: 32 with float exchanges for
+// 256 <= n <= 2048, else 16; 16 forces 16)
+hipError_t launch_col_screen(long n, float2* data, int D, const float2* tfp, const float2* tf, const float2* tw,
+                             const ScreenArgs& sa, const double* vq, int slot0, hipStream_t st, int e = 0);
+// Sampled phase planes (FP64 quarter turns) of layers slot_layer[0 ..
+// nslots) for every direction: vq[((slot D + d) n + j) n + i].
+hipError_t launch_layer_phase(double* vq, const float* masters, const LayerSample* ls, const int* slot_layer,
+                              int nslots, long n, long M, long L, long D, hipStream_t st);
+bool col_screen_supported(long n);
 // Column pass: same transforms along y; no screen.
 // F fields, or (wcnt > 0) F = D * wcnt fields: wavelengths wlo .. wlo + wcnt - 1
 // of each direction.  c: columns per workgroup (0 = default for n); e = 32: the Fresnel pass

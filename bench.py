@@ -160,6 +160,10 @@ def main():
                     "streams (0 = default, 2; 1 = one stream)")
     ap.add_argument("--col-e", type=int, default=0, help="elements per thread of the column pass (0 = default, "
                     "16, 32)")
+    ap.add_argument("--col2-e", type=int, default=0, help="elements per thread of the column passes that apply "
+                    "a screen (0 = default, 16, 32)")
+    ap.add_argument("--gap-passes", type=int, default=0, help="passes over the fields per Fresnel gap (0 = "
+                    "default, 1: x / y order alternating; 2: a row and a column pass)")
     ap.add_argument("--graph", type=int, default=1, help="1: frames replay a captured HIP graph of their kernels "
                     "(0: eager launches)")
     ap.add_argument("--host-copy-steps", type=int, default=20,
@@ -188,6 +192,10 @@ def main():
         ctx.set_tuning("col_e", args.col_e)
     if args.streams:
         ctx.set_tuning("streams", args.streams)
+    if args.gap_passes:
+        ctx.set_tuning("gap_passes", args.gap_passes)
+    if args.col2_e:
+        ctx.set_tuning("col2_e", args.col2_e)
     if args.graph:
         ctx.set_tuning("graph", args.graph)
     if args.calib_copy_mib > 0:
@@ -227,7 +235,7 @@ def main():
         for _ in range(min(args.steps, 20)):
             sim.frame(sync=False)
         ctx.sync()
-    prof = {k: sim.profile_read(k) for k in ("row", "col", "extract", "block")}
+    prof = {k: sim.profile_read(k) for k in ("row", "col", "row2", "col2", "phase", "extract", "block")}
     sim.profile(False)
     pcie = None
     if args.host_copy_steps > 0:
@@ -293,27 +301,61 @@ def main():
                       "how": "one simulation per wavelength, one field per launch, same masters and kernels"}
 
     N = wl["N"]
-    # algorithmic bytes per launch.  Column pass: one c64 read + write of every
-    # field.  Row pass of gap g: the same, except that the top group's pass
-    # creates the field (write only), plus one f32 read of the footprint of
-    # each layer of group g per direction (the last group is applied by the
-    # extraction).  Averaged over the G launches of a frame.
-    sli = np.asarray(setup["sli"])
-    lcount_last = int((sli == sli[0]).sum())  # the ground group holds layer 0
-    col_bytes = 16.0 * N * N * F
-    row_bytes = ((G - 1) * 16.0 + 8.0) * N * N * F / G + 4.0 * N * N * wl["D"] * (wl["L"] - lcount_last) / G
-    avg = {k: (ms / n if n else float("nan")) for k, (ms, n) in prof.items()}
-    dom = "row" if prof["row"][0] >= prof["col"][0] else "col"
-    dom_bytes = row_bytes if dom == "row" else col_bytes
+    kinds = ("row", "col", "row2", "col2", "phase")
+    # Algorithmic bytes of every Fresnel pass of a frame, by kind: one c64
+    # read + write of every field (the first pass creates the field: write
+    # only) and, per direction, one f32 read of the footprint of each layer
+    # whose screen a row pass applies, or one f64 read of the sampled phase
+    # plane of each layer a column pass applies (the phase kernel reads the
+    # f32 footprint and writes the f64 plane).  Layer groups top -> bottom as
+    # the library forms them; the last group is applied by the extraction.
+    sli = list(np.asarray(setup["sli"]))
+    L = wl["L"]
+    lcount = []
+    for l in range(L - 1, -1, -1):
+        if l == L - 1 or sli[l + 1] != sli[l]:
+            lcount.append(1)
+        else:
+            lcount[-1] += 1
+    fused = ctx.get_tuning("gap_passes") == 1 and N <= 4096 and G >= 2
+    field_rw, field_w, foot = 16.0 * N * N * F, 8.0 * N * N * F, 4.0 * N * N * wl["D"]
+    passes = []  # (kind, bytes)
+    if fused:
+        nslots = sum(lcount[k] for k in range(1, G, 2))
+        passes.append(("phase", 3.0 * foot * nslots))
+        for k in range(G + 1):
+            rows = k % 2 == 0
+            if k == 0:
+                passes.append(("row", field_w + foot * lcount[0]))
+            elif k == G:
+                passes.append(("row" if rows else "col", field_rw))
+            elif rows:
+                passes.append(("row2", field_rw + foot * lcount[k]))
+            else:
+                passes.append(("col2", field_rw + 2.0 * foot * lcount[k]))
+    else:
+        for g in range(G):
+            passes.append(("row", (field_w if g == 0 else field_rw) + foot * lcount[g]))
+            passes.append(("col", field_rw))
+    kb = {k: sum(b for kk, b in passes if kk == k) for k in kinds}
+    kn = {k: sum(1 for kk, _ in passes if kk == k) for k in kinds}
+    bytes_per_launch = {k: kb[k] / kn[k] for k in kinds if kn[k]}
+    avg = {k: ms / n for k, (ms, n) in prof.items() if n}
+    dom = max(kinds, key=lambda k: prof[k][0])
+    dom_bytes = bytes_per_launch[dom]
     achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
-    fft_ms = prof["row"][0] + prof["col"][0]
-    fft_gbs = (row_bytes * prof["row"][1] + col_bytes * prof["col"][1]) / (fft_ms * 1e-3) / 1e9
+    fft_ms = sum(prof[k][0] for k in kinds)
+    fft_gbs = sum(bytes_per_launch[k] * prof[k][1] for k in kinds if prof[k][1]) / (fft_ms * 1e-3) / 1e9
     value = args.steps * F / elapsed
-    # whole-frame algorithmic bytes per wavefront-frame: two c64 passes per gap,
-    # one f32 read of each layer's footprint, an f32 OPD and amplitude write
-    frame_bytes = 32.0 * G * N * N + 4.0 * wl["L"] * N * N + 8.0 * wl["Nout"] ** 2
+    # whole-frame algorithmic bytes per wavefront-frame: the passes above,
+    # the extraction's c64 read of the field, an f32 OPD and amplitude write
+    frame_bytes = (sum(b for _, b in passes) + 8.0 * N * N * F) / F + 8.0 * wl["Nout"] ** 2
     kernel_names = {"row": "k_row_fresnel<2048> (x pass: screen phase, FFT, transfer function, IFFT)",
-                    "col": "k_col<2048> (y pass: FFT, transfer function, IFFT)"}
+                    "col": "k_col<2048> (y pass: FFT, transfer function, IFFT)",
+                    "row2": "k_row_fresnel<2048, PRE> (x pass: FFT, transfer function, IFFT, screen phase, "
+                            "FFT, transfer function, IFFT)",
+                    "col2": "k_col_screen_split<2048, 8, 32> (y pass: FFT, transfer function, IFFT, screen "
+                            "phase, FFT, transfer function, IFFT)"}
     traffic = None
     tr = read_traffic()
     if tr and dom in tr:
@@ -347,6 +389,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_launch": dom_bytes,
+            "launches_per_frame": kn[dom],
             "avg_launch_ms": round(avg[dom], 5),
         },
         "fft_hbm_gbs": round(fft_gbs, 1),
@@ -360,6 +403,7 @@ def main():
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c"),
                    "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),
+                   "gap_passes": ctx.get_tuning("gap_passes"), "col2_e": ctx.get_tuning("col2_e"),
                    "graph": ctx.get_tuning("graph")},
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
     }

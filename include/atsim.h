@@ -70,6 +70,18 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             after set k - 1's first row pass) and join before the
  *             extraction; 1: one stream.  Frames under atsim_sim_profile use
  *             one stream.
+ *   "gap_passes" 1 (default): one pass over the fields per Fresnel gap --
+ *             the x and y factors of a gap commute, so gaps alternate
+ *             between x-first and y-first and a row (column) pass applies
+ *             the x (y) factor of the gap before a layer group, the group's
+ *             screen and the x (y) factor of the gap after it (n <= 4096,
+ *             at least 2 gaps; the screens of the column passes' layers are
+ *             sampled once per frame and direction into FP64 planes of
+ *             n^2 x 8 bytes each); 2: a row pass (screen, x factor) and a
+ *             column pass (y factor) per gap
+ *   "col2_e"  elements per thread of the column passes that apply a screen
+ *             (16, or 32: 8 columns per workgroup exchanging through half
+ *             the LDS, the default for 256 <= n <= 2048)
  *   "graph"   1: atsim_sim_frame replays a HIP graph of the frame's kernels,
  *             captured at the second frame (and again when the output
  *             pointers change); the position upload stays outside the graph
@@ -195,11 +207,14 @@ int atsim_sim_frame(atsim_sim *sim, float *d_pha, float *d_amp, void *d_field);
 int atsim_sim_set_positions(atsim_sim *sim, const double *xpos, const double *ypos);
 /* current FP64 positions [D][L] (host arrays, nullable) */
 int atsim_sim_positions(atsim_sim *sim, double *xpos, double *ypos);
-/* number of Fresnel gaps per frame (row + column pass each) */
+/* number of Fresnel gaps per frame */
 long atsim_sim_gaps(atsim_sim *sim);
 /* Per-launch HIP-event timing of the frame kernels, for benchmarking.
- * enable = 1 starts (and clears) the record, 0 stops it.  kind: 0 row pass,
- * 1 column pass, 2 extraction, 3 block unwrap; returns the summed duration
+ * enable = 1 starts (and clears) the record, 0 stops it.  kind: 0 row pass
+ * with one factor (screen + x factor, or x factor alone), 1 column pass with
+ * one factor, 2 extraction, 3 block unwrap, 4 row pass with two x factors
+ * and a screen, 5 column pass with two y factors and a screen, 6 sampling of
+ * the phase planes that column passes apply; returns the summed duration
  * [ms] and the number of launches recorded since the last enable. */
 int atsim_sim_profile(atsim_sim *sim, int enable);
 int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
