@@ -513,38 +513,56 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
     for (int q = 0; q < E; q++) buf_store2(v[q], fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
 }
 
-// Sampled phase of layer slot_layer[slot] for direction d, row j, in FP64
-// quarter turns (x 2 / pi): vq[((slot D + d) N + j) N + i].  The 4 master rows
-// of the row are staged in LDS and every pixel sums its 4 x 4 taps in the
-// order of the row pass (x taps per master row, then the rows).
+// Sampled phase of layer slot_layer[slot] for direction d, in FP64 quarter
+// turns (x 2 / pi): vq[((slot D + d) N + j) N + i].  A workgroup makes PR
+// consecutive rows j0 .. j0 + PR - 1 from master rows y0 + j0 - 1 ..
+// y0 + j0 + PR + 2 (PR + 3 <= 8 staged rows); every pixel sums its 4 x 4
+// taps in the order of the row pass (x taps per master row, then the rows).
+constexpr int PHASE_ROWS = 4;
 __global__ void __launch_bounds__(256) k_layer_phase(double* __restrict__ vq, const float* __restrict__ masters,
                                                      const LayerSample* __restrict__ ls,
                                                      const int* __restrict__ slot_layer, int N, int M, int L) {
+    constexpr int PR = PHASE_ROWS;
+    static_assert(PR + 3 <= 8, "two 4-row stagings");
     extern __shared__ float2 lds[];
-    float* S = reinterpret_cast<float*>(lds);  // [4][N + 4]
-    // 4 adjacent rows (which stage overlapping master rows) on one XCD
-    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, slot = blockIdx.z, D = gridDim.y;
+    float* S = reinterpret_cast<float*>(lds);  // [8][N + 4]
+    // 4 adjacent row blocks (which stage overlapping master rows) on one XCD
+    const int nb = N / PR;
+    const int j0 = xcd_swizzle(blockIdx.x, nb) * PR, d = blockIdx.y, slot = blockIdx.z, D = gridDim.y;
     const int SW = N + 4;
     const int l = slot_layer[slot];
     const LayerSample* p = ls + (size_t)d * L + l;
     const float* m = masters + (size_t)l * M * M;
-    const int ys = mod_m(p->y0 + j - 1, M), xs = mod_m(p->x0 - 1, M);
-    stage_rows4<8>(S, SW, m, M, ys, xs, N, threadIdx.x, blockDim.x);
-    stage_tail(S, SW, m, 0, M, &ys, &xs, 1, N, 3, threadIdx.x, blockDim.x);
+    int ys[2];
+    ys[0] = mod_m(p->y0 + j0 - 1, M);
+    ys[1] = mod_m(p->y0 + j0 + 3, M);
+    int xs[2];
+    xs[0] = xs[1] = mod_m(p->x0 - 1, M);
+    stage_rows4<4>(S, SW, m, M, ys[0], xs[0], N, threadIdx.x, blockDim.x);
+    stage_rows4<4>(S + 4 * SW, SW, m, M, ys[1], xs[1], N, threadIdx.x, blockDim.x);
+    stage_tail(S, SW, m, 0, M, ys, xs, 2, N, 3, threadIdx.x, blockDim.x);
     __syncthreads();
-    double* out = vq + (((size_t)slot * D + d) * N + j) * N;
+    // per column: the x-filtered value of each staged row once, then the PR
+    // outputs from 4 of them each (the same FP64 operations in the same order
+    // as one pixel at a time)
+    double* out = vq + (((size_t)slot * D + d) * N + j0) * N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        double v = 0.0;
+        double h[PR + 3];
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const float* r = S + b * SW + i;
-            double h = p->wx[0] * (double)r[0];
-            h = fma(p->wx[1], (double)r[1], h);
-            h = fma(p->wx[2], (double)r[2], h);
-            h = fma(p->wx[3], (double)r[3], h);
-            v = fma(p->wy[b], h, v);
+        for (int r = 0; r < PR + 3; r++) {
+            const float* q = S + r * SW + i;
+            double x = p->wx[0] * (double)q[0];
+            x = fma(p->wx[1], (double)q[1], x);
+            x = fma(p->wx[2], (double)q[2], x);
+            h[r] = fma(p->wx[3], (double)q[3], x);
         }
-        out[i] = v * 0.63661977236758134308;
+#pragma unroll
+        for (int rr = 0; rr < PR; rr++) {
+            double v = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) v = fma(p->wy[b], h[rr + b], v);
+            out[(size_t)rr * N + i] = v * 0.63661977236758134308;
+        }
     }
 }
 
@@ -795,8 +813,9 @@ hipError_t launch_layer_phase(double* vq, const float* masters, const LayerSampl
         allow_dynamic_lds((const void*)k_layer_phase);
         attr = true;
     }
-    dim3 grid((unsigned)N, (unsigned)D, (unsigned)nslots);
-    k_layer_phase<<<grid, 256, (size_t)4 * (N + 4) * sizeof(float), st>>>(vq, masters, ls, slot_layer, (int)N,
+    if (N % PHASE_ROWS) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(N / PHASE_ROWS), (unsigned)D, (unsigned)nslots);
+    k_layer_phase<<<grid, 256, (size_t)8 * (N + 4) * sizeof(float), st>>>(vq, masters, ls, slot_layer, (int)N,
                                                                            (int)M, (int)L);
     return hipGetLastError();
 }
