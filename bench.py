@@ -35,6 +35,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "wavefront-frames/sec (λ×dir) at 2048², 10 layers; FFT HBM GB/s; err vs FP64"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 FP32 / FP64 instruction per
+# 4 cycles on a 16-lane SIMD, 2.4 GHz
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def headline_workload():
@@ -70,21 +73,37 @@ def make_sim(ctx, wl, M=None, lambdas=None, unwrap=None):
     return sim, dict(sa=sa, sli=sli, x0=x0, y0=y0, vx=vx, vy=vy, scoeff=scoeff, unwrap=np.asarray(unwrap))
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprof PMC summary"""
-    p = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+def read_pass_table(fname, column):
+    """{pass: value} from the two-column table headed `| pass | <column> |` of a
+    committed rocprof PMC summary under profiles/"""
+    p = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(p):
         return None
-    out = {}
+    out, inside = {}, False
     for line in open(p):
-        if line.startswith("|") and "bytes_per_launch" not in line and "---" not in line:
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 2:
-                try:
-                    out[cells[0]] = float(cells[-1])
-                except ValueError:
-                    pass
+        cells = [c.strip() for c in line.strip().strip("|").split("|")] if line.startswith("|") else None
+        if cells is None:
+            inside = False
+            continue
+        if cells == ["pass", column]:
+            inside = True
+            continue
+        if inside and len(cells) == 2 and not cells[1].startswith("-"):
+            try:
+                out[cells[0]] = float(cells[1])
+            except ValueError:
+                pass
     return out or None
+
+
+def read_traffic():
+    """HBM bytes per launch of each pass from the committed FETCH_SIZE / WRITE_SIZE summary"""
+    return read_pass_table("hbm_traffic.md", "bytes_per_launch")
+
+
+def read_valu():
+    """VALU wave instructions per launch of each pass from the committed SQ_INSTS_VALU summary"""
+    return read_pass_table("valu_headline_2048_10L_8lam.md", "wave_instructions_per_launch")
 
 
 def cpu_baseline_and_error(ctx, wl, masters_gpu, frames=2, threads=0):
@@ -360,6 +379,12 @@ def main():
     tr = read_traffic()
     if tr and dom in tr:
         traffic = tr[dom]
+    valu = None
+    vi = read_valu()
+    if vi and dom in vi:
+        rate = vi[dom] / (avg[dom] * 1e-3)
+        valu = {"wave_instructions_per_launch": vi[dom], "achieved_per_s": round(rate, -6),
+                "peak_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
     line = {
         "metric": METRIC,
@@ -390,6 +415,10 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": dom_bytes,
             "launches_per_frame": kn[dom],
+            # the fused passes are bound by VALU issue (FP32 FFT arithmetic)
+            # before HBM: wave instructions per launch (SQ_INSTS_VALU) over
+            # the live launch duration, against the issue peak
+            "valu": valu,
             "avg_launch_ms": round(avg[dom], 5),
         },
         "fft_hbm_gbs": round(fft_gbs, 1),
