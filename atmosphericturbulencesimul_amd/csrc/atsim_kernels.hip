@@ -6,6 +6,14 @@
 //                  WFpropagate.c:68-152); k_row can also multiply the field by
 //                  the screen phase of a group of layers first, sampling the
 //                  master screens itself (AtmosphericTurbulence.c:2759-3412).
+//  k_row_fresnel   the frame's x pass over several wavelengths of one row:
+//                  optionally the x factor of the gap before a layer group,
+//                  the group's screen, the x factor of the gap after it.
+//  k_col_screen(_split)  the frame's y pass: y factor, screen (from sampled
+//                  phase planes), y factor.  With the x and y factors of a
+//                  gap commuting, gaps alternate x-first / y-first and a
+//                  frame makes one pass over the fields per gap.
+//  k_layer_phase   FP64 phase planes of the layers column passes apply.
 //  k_permut        quadrant swap (fft.c:570-792).
 //  k_noise         Philox4x32-10 + Box-Muller complex Gaussian noise.
 //  k_synth_filter  von Karman spectral filter on the noise, written in the

@@ -16,6 +16,9 @@
   8 wavelengths and 16 directions on a 4 x 4 grid at -1, -1/3, 1/3, 1 arcmin:
   energy of all 128 fields at every size, FP64 positions, and at 512^2 the
   two corner directions against the oracle.
+* 4096^2, 6 layers (5 gaps), 3 wavelengths, 8192^2 GPU masters: one pass per
+  gap against two passes per gap (relative L2 2e-6) and the energy N^2 of
+  every field.
 
 Bars as in test_gpu_sim: relative L2 1e-5 for fields and amplitudes; phase
 relative L2 1e-5 on agreeing 2 pi decisions, at most 0.1 % disagreeing."""
@@ -58,6 +61,48 @@ def _gpu_masters_case(ctx, N, M, alts, cn2, speeds, dirs, lambdas, lam_ref, dx, 
     sim.synth_masters(seed, L0 / dx, l0 / dx)
     sim.normalize_masters(r0 / dx, cn2)
     return sim, dict(sa=sa, sli=sli, x0=x0, y0=y0, vx=vx, vy=vy, scoeff=scoeff, unwrap=unwrap, lambdas=lambdas)
+
+
+def test_4096_one_pass_per_gap():
+    # at 4096^2 the passes run the 16-value column kernel (4 columns per
+    # workgroup) and the row kernel with 2 wavelengths per workgroup: one pass
+    # per gap against two passes per gap (same Philox masters), and the
+    # energy N^2 of every field
+    import torch
+    from atmosphericturbulencesimul_amd.api import Context
+    print("case: 4096^2, M=8192, 6 layers (5 gaps), 3 wavelengths, 2 frames", flush=True)
+    N, M, dx = 4096, 8192, 0.01
+    alts = [0.0, 1500.0, 3000.0, 5000.0, 8000.0, 12000.0]
+    cn2 = [0.35, 0.2, 0.15, 0.12, 0.1, 0.08]
+    speeds = np.linspace(5.0, 30.0, 6)
+    dirs = np.random.default_rng(0xA75 + 5000).uniform(0.0, 2 * math.pi, 6)
+    lams = [0.5e-6, 1.1e-6, 2.2e-6]
+    sims, ctxs = [], []
+    for gp in (1, 2):
+        cx = Context(0)
+        cx.set_tuning("gap_passes", gp)
+        ctxs.append(cx)
+        sim, _ = _gpu_masters_case(cx, N, M, alts, cn2, speeds, dirs, lams, 0.5e-6, dx, 0.12, 20.0, 0.01,
+                                   seed=0xA75 + 5000)
+        sims.append(sim)
+    assert sims[0].gaps() == 5
+    fields = [torch.empty((1, 3, N, N), dtype=torch.complex64, device="cuda") for _ in sims]
+    for k in range(2):
+        outs = [s.frame(f) for s, f in zip(sims, fields)]
+        e = (outs[0][1].double() ** 2).sum(dim=(-1, -2)).cpu().numpy().ravel() / (N * N)
+        print(f"frame {k}: energy / N^2 - 1 =", np.abs(e - 1).max(), flush=True)
+        assert np.abs(e - 1).max() < 1e-5
+        for w in range(3):
+            a, b = fields[0][0, w], fields[1][0, w]
+            rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+            ra = float(torch.linalg.vector_norm(outs[0][1][0, w] - outs[1][1][0, w]) /
+                       torch.linalg.vector_norm(outs[1][1][0, w]))
+            print(f"frame {k} w {w}: field {rel:.2e} amp {ra:.2e}", flush=True)
+            assert rel < 2e-6 and ra < 2e-6
+    for s in sims:
+        s.close()
+    for cx in ctxs:
+        cx.close()
 
 
 def test_1024_seven_layers_five_wavelengths(ctx, oracle):
