@@ -1305,8 +1305,16 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         const bool block_rule = (umask >> w) & 1;
         float cx = 0.f, cy = 0.f;  // this pixel's contribution to its block sums
         if (valid) {
+            // The last group's screen only rotates the field: arg(E e^{i PL})
+            // is arg E + PL modulo 2 pi and |E| does not change, so the
+            // rotation is applied to the FP64 phase (one add) unless the
+            // rotated field itself is an output.
             float2 E = Ein[w];
-            if (a.amplitude && a.last_count > 0) E = cmul(E, expi_d(PL[w]));
+            double rot = 0.0;
+            if (a.amplitude && a.last_count > 0) {
+                if (FULL && a.field_out) E = cmul(E, expi_d(PL[w]));
+                else rot = PL[w];
+            }
             if (FULL && a.field_out) a.field_out[((size_t)f * a.N + jf) * a.N + ifl] = E;
             if (rowin && colin) {
                 const size_t oi = o0 + (size_t)w * plane;
@@ -1317,9 +1325,14 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                     // |E| with the hardware square root (1 ulp); arg E with
                     // the polynomial arctangent (< 4e-7 rad)
                     a.amp[oi] = __builtin_amdgcn_sqrtf(E.x * E.x + E.y * E.y);
-                    const double ph = (double)atan2_poly(E.y, E.x);
+                    // arg of the rotated field, not yet brought into (-pi, pi]
+                    const double ph = (double)atan2_poly(E.y, E.x) + rot;
                     if (!block_rule) {
-                        a.pha[oi] = (float)(ph + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
+                        // this rule adds whole turns of PHI alone to the phase
+                        // in (-pi, pi] (AT.c:3456), so the rotated phase is
+                        // wrapped first
+                        const double phw = fma(-6.283185307179586477, rint(ph * inv2pi), ph);
+                        a.pha[oi] = (float)(phw + 2.0 * M_PI * unwrap_count(PHI[w] * inv2pi));
                     } else {
                         // whole turns of PHI - atan2 go into the phase now, the
                         // FP64 remainder (rounded to float) waits for the
