@@ -619,7 +619,8 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     double* V = reinterpret_cast<double*>(S + 4 * SW);  // [SEG] sampled screen
     const int mwc = blockDim.x / T;                      // wavelengths per workgroup (<= MW)
     const int wend = sa.wcnt > 0 ? sa.wlo + sa.wcnt : sa.W;  // end of this launch's wavelength range
-    const int j = xcd_swizzle(blockIdx.x, N), d = blockIdx.y, w0 = (sa.wcnt > 0 ? sa.wlo : 0) + blockIdx.z * mwc;
+    // 16 adjacent rows (which stage overlapping master rows) run on one XCD
+    const int j = xcd_chunk<16>(blockIdx.x, N), d = blockIdx.y, w0 = (sa.wcnt > 0 ? sa.wlo : 0) + blockIdx.z * mwc;
     const int nw = min(mwc, wend - w0);
     const int lw = threadIdx.x / T, t = threadIdx.x - lw * T;
     const bool active = lw < nw;
