@@ -34,6 +34,7 @@ struct atsim_ctx {
     int gap_passes = 1;  // 1: one pass over the field per gap (x / y order alternating), 2: a row and a column pass
     int col2_e = 0;      // elements per thread of the column passes with a screen (0 = default, 16, 32)
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
+    unsigned tuning_gen = 0;  // bumped by every accepted atsim_set_tuning: captured frame graphs are stale
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ready[ATSIM_COPY_SLOTS] = {};
@@ -251,6 +252,7 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else {
         return fail(c, ATSIM_EINVAL, "unknown tuning key '%s'", key);
     }
+    c->tuning_gen++;
     return ATSIM_OK;
 }
 
@@ -556,6 +558,7 @@ struct atsim_sim {
     // captured frame graphs, keyed by the output pointers they were captured
     // with (two sets: double-buffered outputs alternate between them)
     long frames_done = 0;
+    unsigned graph_gen = 0;  // the context's tuning generation the captured graphs belong to
     struct GraphSlot {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -1031,7 +1034,16 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     // graph mode: from the second frame on (the first one sets the kernels'
     // LDS attributes), replay the captured kernel sequence; per-launch
     // profiling needs the eager path
-    if (!c->graph || s->prof || s->frames_done == 0) {
+    // a tuning change since the graphs were captured: drop them, run this
+    // frame eagerly (first launches of other kernel variants set their LDS
+    // attributes outside a capture) and capture again at the next one
+    bool retuned = false;
+    if (s->graph_gen != c->tuning_gen) {
+        sim_drop_graph(s);
+        s->graph_gen = c->tuning_gen;
+        retuned = true;
+    }
+    if (!c->graph || s->prof || s->frames_done == 0 || retuned) {
         s->frames_done++;
         return enqueue_frame(s, d_pha, d_amp, d_field);
     }

@@ -84,7 +84,8 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             the LDS, the default for 256 <= n <= 2048)
  *   "graph"   1: atsim_sim_frame replays a HIP graph of the frame's kernels,
  *             captured at the second frame (and again when the output
- *             pointers change); the position upload stays outside the graph
+ *             pointers change, or at the second frame after a tuning
+ *             change); the position upload stays outside the graph
  * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
 int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
 int atsim_get_tuning(atsim_ctx *ctx, const char *key);

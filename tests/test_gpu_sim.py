@@ -297,6 +297,35 @@ def test_one_pass_per_gap(oracle, nl, bicubic, bin_m, two_dirs, streams, N, col2
     cx.close()
 
 
+def test_tuning_change_between_graph_frames(oracle):
+    # with graph replay, frames after a tuning change run the new kernel
+    # sequence: the same frames as a context tuned that way from the start
+    from atmosphericturbulencesimul_amd.api import Context
+    c = _many_gaps_case(oracle, 6, N=256, Nout=228, M=512)
+    runs = []
+    for switch in (True, False):
+        cx = Context(0)
+        cx.set_tuning("graph", 1)
+        if not switch:
+            cx.set_tuning("gap_passes", 2)
+            cx.set_tuning("streams", 1)
+        gs = gpu_sim(cx, c)
+        frames = []
+        for k in range(6):
+            if switch and k == 3:
+                cx.set_tuning("gap_passes", 2)
+                cx.set_tuning("streams", 1)
+            pha, amp = gs.frame()
+            frames.append((pha.cpu().numpy().copy(), amp.cpu().numpy().copy()))
+        runs.append(frames)
+        cx.close()
+    for k in range(3, 6):
+        assert np.array_equal(runs[0][k][0], runs[1][k][0]) and np.array_equal(runs[0][k][1], runs[1][k][1])
+    # before the switch the frames came from one pass per gap: close, not equal
+    assert rel_l2(runs[0][2][1], runs[1][2][1]) < 2e-6
+    assert not np.array_equal(runs[0][2][1], runs[1][2][1])
+
+
 @pytest.mark.parametrize("case", ["identity", "isolated_block", "branch_cut"])
 def test_block_fixup_against_the_oracle(ctx, oracle, case):
     # the GPU fix-up heuristic on block sums from planted fields: nothing to
