@@ -537,7 +537,7 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
 // y0 + j0 + PR + 2 (PR + 3 <= 8 staged rows); every pixel sums its 4 x 4
 // taps in the order of the row pass (x taps per master row, then the rows).
 constexpr int PHASE_ROWS = 4;
-__global__ void __launch_bounds__(256) k_layer_phase(double* __restrict__ vq, const float* __restrict__ masters,
+__global__ void __launch_bounds__(512) k_layer_phase(double* __restrict__ vq, const float* __restrict__ masters,
                                                      const LayerSample* __restrict__ ls,
                                                      const int* __restrict__ slot_layer, int N, int M, int L) {
     constexpr int PR = PHASE_ROWS;
@@ -835,7 +835,7 @@ hipError_t launch_layer_phase(double* vq, const float* masters, const LayerSampl
     }
     if (N % PHASE_ROWS) return hipErrorInvalidValue;
     dim3 grid((unsigned)(N / PHASE_ROWS), (unsigned)D, (unsigned)nslots);
-    k_layer_phase<<<grid, 256, (size_t)8 * (N + 4) * sizeof(float), st>>>(vq, masters, ls, slot_layer, (int)N,
+    k_layer_phase<<<grid, 512, (size_t)8 * (N + 4) * sizeof(float), st>>>(vq, masters, ls, slot_layer, (int)N,
                                                                            (int)M, (int)L);
     return hipGetLastError();
 }
