@@ -17,9 +17,14 @@
  *                           field invariance, closed-form phase of an integer-
  *                           frequency plane wave, and an independent numpy
  *                           angular-spectrum implementation.
- *   orc_fresnel_init_tf /
- *   orc_fresnel1 / cube     closed form with FPMASKRAD covering the grid and a
- *                           4x4 sub-sample sum evaluated independently in numpy.
+ *   orc_fresnel_init_tf     the separable closed form of the 4x4 sub-sample
+ *                           average (pairs of offsets +-0.375, +-0.125 give
+ *                           cos factors) with the 0.3 cut-off, the small-
+ *                           coeff phase -coeff (r^2 + 0.15625), and at z = 0
+ *                           the mask rule "at least 5 of 16 sub-samples
+ *                           strictly inside FPMASKRAD" over many rim radii.
+ *   orc_fresnel1 / cube     the exact table reproduces orc_fresnel; cube
+ *                           slices equal single propagations.
  *   orc_philox4x32_10       Random123 known-answer vectors.
  *   orc_noise_c64           moments (mean 0, var 1 per component), independence.
  *   orc_vk_filter           closed form at chosen pixels, DC = 0.
@@ -39,6 +44,16 @@
  *   orc_extract_simple      identity when |PHI - pha| < pi; 2 pi multiples.
  *   orc_extract_block       identity on smooth fields; hand-built block
  *                           inconsistency fixed by the +-2 pi heuristic.
+ *   orc_block_fix           first-round counts equal a brute-force enumeration
+ *                           of all block pairs at Chebyshev distance 1 (0.2)
+ *                           and 2 (0.15) on random grids; a single planted
+ *                           pair at every offset and grid position; the plim
+ *                           rule (max - 0.001, floor 0.5) on hand-built count
+ *                           maps; the interior rule and the five quiet rounds
+ *                           from oscillating blocks inside / outside the
+ *                           interior (1000 / 5 rounds); float counts in the
+ *                           reference's per-block step order
+ *                           (tests/test_oracle_block.py).
  */
 #include "atsim_oracle.h"
 
@@ -767,40 +782,28 @@ static const struct { int dx1, dy1, dx2, dy2, xm, ym; double c; } BLK_PAIRS[12] 
     {0, 2, 2, 0, 2, 2, 0.15}, {0, 1, 2, 0, 2, 1, 0.15}, {2, 1, 0, 0, 2, 1, 0.15},
 };
 
-/* AT.c:3523-4262. */
-long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
-                       int prec_f32, double *pha, double *amp, double *peak_out)
+/* AT.c:3627-4216: the iterative +-2 pi consistency fix-up on an xs x ys grid
+ * of block peaks bin[jj2 * xs + ii2], in place.  Each round clears the
+ * counts, adds +-c for every relation of BLK_PAIRS whose peaks differ by more
+ * than pi (the block with the larger peak gets -c, the other +c), in the
+ * reference's loop order (relation, then ii2, then jj2), sets
+ * plim = max(max|count| - 0.001, 0.5) and moves every block whose |count|
+ * exceeds plim by -+2 pi; a round that moves no interior block (ii2, jj2 in
+ * [2, size - 3]) counts towards the five that end the loop.  With prec_f32
+ * the counts and peaks are floats as with WFprecision = 0 (every += rounds).
+ * ch_first (nullable, xs * ys) receives the first round's counts.
+ * max_rounds caps the loop (the reference's cap is 1000; a smaller one stops
+ * the loop early for tests).  Returns the number of rounds. */
+long orc_block_fix(double *bin, long xs, long ys, int prec_f32, double *ch_first, long max_rounds)
 {
-    long o = (N - Nout) / 2;
-    long xs = Nout / 20, ys = Nout / 20;
     long nb = xs * ys;
-    double *bre = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
-    double *bim = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
-    double *bin = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
     double *ch = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
-    for (long ii = 0; ii < Nout; ii++)
-        for (long jj = 0; jj < Nout; jj++) {
-            long k = (jj + o) * N + ii + o;
-            double re = E[2 * k], im = E[2 * k + 1];
-            amp[jj * Nout + ii] = sqrt(re * re + im * im);
-            double p = atan2(im, re);
-            pha[jj * Nout + ii] = p;
-            long ii2 = (long)(1.0 * ii / Nout * xs);
-            long jj2 = (long)(1.0 * jj / Nout * ys);
-            if (ii2 < xs && jj2 < ys) {
-                long b = jj2 * xs + ii2;
-                bre[b] += cos(PHI[k] - p);
-                bim[b] += sin(PHI[k] - p);
-                if (prec_f32) { bre[b] = rf(bre[b]); bim[b] = rf(bim[b]); }
-            }
-        }
-    for (long b = 0; b < nb; b++) {
-        bin[b] = atan2(bim[b], bre[b]);
-        if (prec_f32) bin[b] = rf(bin[b]);
-    }
+    if (prec_f32)
+        for (long b = 0; b < nb; b++) bin[b] = rf(bin[b]);
     long chcnt = 1, chcnt0cnt = 0, chiter = 0;
     double plim = 1.0;
-    while (plim > 0.51 && chiter < 1000 && chcnt0cnt < 5) {
+    if (max_rounds <= 0 || max_rounds > 1000) max_rounds = 1000;
+    while (plim > 0.51 && chiter < max_rounds && chcnt0cnt < 5) {
         chiter++;
         chcnt = 0;
         for (long b = 0; b < nb; b++) ch[b] = 0.0;
@@ -811,11 +814,22 @@ long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
                     long i1 = (jj2 + BLK_PAIRS[q].dy1) * xs + ii2 + BLK_PAIRS[q].dx1;
                     long i2 = (jj2 + BLK_PAIRS[q].dy2) * xs + ii2 + BLK_PAIRS[q].dx2;
                     double pv1 = bin[i1], pv2 = bin[i2];
-                    if (pv2 > pv1 + M_PI) { ch[i1] += c; ch[i2] -= c; }
-                    if (pv1 > pv2 + M_PI) { ch[i1] -= c; ch[i2] += c; }
-                    if (prec_f32) { ch[i1] = rf(ch[i1]); ch[i2] = rf(ch[i2]); }
+                    if (pv2 > pv1 + M_PI) {
+                        ch[i1] += c;
+                        if (prec_f32) ch[i1] = rf(ch[i1]);
+                        ch[i2] -= c;
+                        if (prec_f32) ch[i2] = rf(ch[i2]);
+                    }
+                    if (pv1 > pv2 + M_PI) {
+                        ch[i1] -= c;
+                        if (prec_f32) ch[i1] = rf(ch[i1]);
+                        ch[i2] += c;
+                        if (prec_f32) ch[i2] = rf(ch[i2]);
+                    }
                 }
         }
+        if (chiter == 1 && ch_first)
+            for (long b = 0; b < nb; b++) ch_first[b] = ch[b];
         plim = 0.0;
         for (long b = 0; b < nb; b++)
             if (fabs(ch[b]) > plim) plim = fabs(ch[b]);
@@ -839,6 +853,41 @@ long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
         if (chcnt == 0) chcnt0cnt++;
         else chcnt0cnt = 0;
     }
+    free(ch);
+    return chiter;
+}
+
+/* AT.c:3523-4262. */
+long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
+                       int prec_f32, double *pha, double *amp, double *peak_out)
+{
+    long o = (N - Nout) / 2;
+    long xs = Nout / 20, ys = Nout / 20;
+    long nb = xs * ys;
+    double *bre = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
+    double *bim = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
+    double *bin = (double *)calloc(nb > 0 ? nb : 1, sizeof(double));
+    for (long ii = 0; ii < Nout; ii++)
+        for (long jj = 0; jj < Nout; jj++) {
+            long k = (jj + o) * N + ii + o;
+            double re = E[2 * k], im = E[2 * k + 1];
+            amp[jj * Nout + ii] = sqrt(re * re + im * im);
+            double p = atan2(im, re);
+            pha[jj * Nout + ii] = p;
+            long ii2 = (long)(1.0 * ii / Nout * xs);
+            long jj2 = (long)(1.0 * jj / Nout * ys);
+            if (ii2 < xs && jj2 < ys) {
+                long b = jj2 * xs + ii2;
+                bre[b] += cos(PHI[k] - p);
+                bim[b] += sin(PHI[k] - p);
+                if (prec_f32) { bre[b] = rf(bre[b]); bim[b] = rf(bim[b]); }
+            }
+        }
+    for (long b = 0; b < nb; b++) {
+        bin[b] = atan2(bim[b], bre[b]);
+        if (prec_f32) bin[b] = rf(bin[b]);
+    }
+    long chiter = orc_block_fix(bin, xs, ys, prec_f32, NULL, 1000);
     double peakpha = 0.0;
     for (long ii = 0; ii < Nout; ii++)
         for (long jj = 0; jj < Nout; jj++) {
@@ -851,6 +900,6 @@ long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
         }
     if (peak_out)
         for (long b = 0; b < nb; b++) peak_out[b] = bin[b];
-    free(bre); free(bim); free(bin); free(ch);
+    free(bre); free(bim); free(bin);
     return chiter;
 }

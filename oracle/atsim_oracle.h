@@ -130,6 +130,11 @@ void orc_extract_simple(const double *E, const double *PHI, long N, long Nout,
  * Returns the iteration count of the fix-up loop. */
 long orc_extract_block(const double *E, const double *PHI, long N, long Nout,
                        int prec_f32, double *pha, double *amp, double *peak_out);
+/* The block fix-up loop alone (AtmosphericTurbulence.c:3627-4216) on an
+ * xs x ys grid of block peaks, in place; ch_first (nullable) receives the
+ * first round's counts; max_rounds <= 1000 caps the loop (1000 in the
+ * reference).  Returns the number of rounds. */
+long orc_block_fix(double *bin, long xs, long ys, int prec_f32, double *ch_first, long max_rounds);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,8 @@ def lib():
         L.orc_extract_simple.argtypes = [_dp, _dp, C.c_long, C.c_long, _dp, _dp]
         L.orc_extract_block.argtypes = [_dp, _dp, C.c_long, C.c_long, C.c_int, _dp, _dp, _dp]
         L.orc_extract_block.restype = C.c_long
+        L.orc_block_fix.argtypes = [_dp, C.c_long, C.c_long, C.c_int, _dp, C.c_long]
+        L.orc_block_fix.restype = C.c_long
         _lib = L
     return _lib
 
@@ -311,3 +313,15 @@ def extract_block(E, PHI, Nout, prec_f32=False):
     peak = np.empty(max(nb, 1))
     it = lib().orc_extract_block(_p(E), _p(PHI), N, Nout, int(prec_f32), _p(pha), _p(amp), _p(peak))
     return pha, amp, peak[:nb].reshape(Nout // 20, Nout // 20), it
+
+
+def block_fix(bin_, prec_f32=False, max_rounds=1000):
+    """The block fix-up loop on a (ys, xs) grid of peaks (AtmosphericTurbulence.c:3627-4216).
+
+    max_rounds <= 1000 stops the loop early (the reference's cap is 1000).
+    Returns (peaks after the loop, first-round counts, number of rounds)."""
+    b = np.array(bin_, dtype=np.float64, order="C", copy=True)
+    ys, xs = b.shape
+    ch = np.zeros_like(b)
+    it = lib().orc_block_fix(_p(b), xs, ys, int(prec_f32), _p(ch), int(max_rounds))
+    return b, ch, it

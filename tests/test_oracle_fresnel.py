@@ -90,3 +90,78 @@ def test_cube(oracle):
         P = oracle.fresnel(E, 0.02, z, 1e-6)
         assert np.allclose(amp[k], np.abs(P), atol=1e-12)
         assert np.allclose(pha[k], np.angle(P), atol=1e-12)
+
+
+def _tf_closed_form(size, coeff):
+    """Closed form of the 4 x 4 sub-sample average of exp(-i coeff r^2) when
+    the mask covers the pixel: the average separates into x and y factors,
+    and pairing the offsets +-0.375 and +-0.125 gives, per axis,
+        a(t) = exp(-i coeff t^2) (exp(-i coeff 0.140625) cos(0.75 coeff t)
+                                  + exp(-i coeff 0.015625) cos(0.25 coeff t)) / 2
+    at the pixel's signed frequency t (the quadrant swap puts t = 0 at index 0)."""
+    t = np.fft.fftfreq(size) * size
+    a = np.exp(-1j * coeff * t ** 2) * 0.5 * (np.exp(-1j * coeff * 0.140625) * np.cos(0.75 * coeff * t)
+                                             + np.exp(-1j * coeff * 0.015625) * np.cos(0.25 * coeff * t))
+    return a[None, :] * a[:, None]
+
+
+@pytest.mark.parametrize("size,dx,z,lam", [(32, 0.02, 2000.0, 1e-6), (64, 0.01, 8000.0, 1.65e-6),
+                                           (48, 0.005, 16000.0, 2.2e-6), (16, 0.05, 500.0, 0.5e-6)])
+def test_init_tf_closed_form_without_mask(oracle, size, dx, z, lam):
+    # FPMASKRAD beyond the corner of the grid: every pixel is the unit phasor
+    # of the closed-form average where its modulus exceeds 0.3, else 0, / size^2
+    C = oracle.fresnel_init_tf(size, dx, z, lam, 10.0 * size)
+    coeff = np.pi * z * lam / (dx * size) ** 2
+    A = _tf_closed_form(size, coeff)
+    amp = np.abs(A)
+    clear = np.abs(amp - 0.3) > 1e-9
+    want = np.where(amp > 0.3, A / np.where(amp > 0, amp, 1.0), 0.0) / size ** 2
+    assert np.allclose(C[clear], want[clear], atol=1e-12 / size ** 2, rtol=0)
+    # the phase of a kept pixel is -coeff (t_x^2 + t_y^2) up to the sub-sample
+    # spread: -coeff 2 <delta^2> = -coeff 0.15625 at small coeff t
+    t = np.fft.fftfreq(size) * size
+    small = (coeff * np.abs(t[None, :]) < 1e-3) & (coeff * np.abs(t[:, None]) < 1e-3)
+    ph = -coeff * (t[None, :] ** 2 + t[:, None] ** 2 + 0.15625)
+    assert np.allclose(np.angle(C[small] * size ** 2 * np.exp(-1j * ph[small])), 0.0, atol=1e-6)
+    # the 0.3 cut-off: where the closed-form modulus is below 0.3 the table is 0
+    assert np.all(C[(amp < 0.3) & clear] == 0)
+
+
+def test_init_tf_aliasing_cutoff_is_reached(oracle):
+    # large coeff: the sub-sample average of the chirp decays with frequency,
+    # so the outer frequencies fall below 0.3 and are zeroed
+    size, dx, z, lam = 64, 0.002, 20000.0, 2.2e-6
+    C = oracle.fresnel_init_tf(size, dx, z, lam, 10.0 * size)
+    A = _tf_closed_form(size, np.pi * z * lam / (dx * size) ** 2)
+    assert np.any(np.abs(A) < 0.29) and np.any(np.abs(A) > 0.31)
+    assert np.array_equal(C != 0, np.abs(A) > 0.3)
+
+
+@pytest.mark.parametrize("size,rad", [(32, 7.3), (32, 11.0), (64, 20.1), (16, 3.4)])
+def test_init_tf_mask_at_zero_distance(oracle, size, rad):
+    # z = 0: every sub-sample adds 1, so the average is k / 16 with k the
+    # sub-samples strictly inside FPMASKRAD; the 0.3 cut-off keeps a pixel
+    # when k >= 5 (4 / 16 = 0.25, 5 / 16 = 0.3125), as 1 / size^2 with no phase
+    C = oracle.fresnel_init_tf(size, 0.01, 0.0, 1e-6, rad)
+    t = np.fft.fftfreq(size) * size
+    k = np.zeros((size, size), int)
+    for ox in (-0.375, -0.125, 0.125, 0.375):
+        for oy in (-0.375, -0.125, 0.125, 0.375):
+            k += (np.hypot(t[None, :] + ox, t[:, None] + oy) < rad)
+    want = np.where(k >= 5, 1.0 / size ** 2, 0.0)
+    assert np.array_equal(C, want.astype(complex))
+    assert np.any((k > 0) & (k < 5))  # the ring of partly covered pixels is exercised
+
+
+def test_init_tf_mask_rim_over_many_radii(oracle):
+    # the strict "distance < FPMASKRAD" test of every sub-sample, over radii
+    # that put sub-samples just inside and just outside the rim
+    size = 32
+    t = np.fft.fftfreq(size) * size
+    offs = (-0.375, -0.125, 0.125, 0.375)
+    d = [np.hypot(t[None, :] + ox, t[:, None] + oy) for ox in offs for oy in offs]
+    radii = sorted({float(v) for v in np.concatenate([x.ravel() for x in d]) if 2 < v < 14})[::7]
+    for rad in [r + e for r in radii for e in (-0.004, 0.004)]:
+        C = oracle.fresnel_init_tf(size, 0.01, 0.0, 1e-6, rad)
+        k = sum((x < rad).astype(int) for x in d)
+        assert np.array_equal(C, np.where(k >= 5, 1.0 / size ** 2, 0.0).astype(complex)), rad
