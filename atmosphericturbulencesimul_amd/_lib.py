@@ -77,6 +77,7 @@ _SIGS = {
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_sim_block_sums": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long]),
     "atsim_block_fixup": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long, C.POINTER(C.c_double)]),
+    "atsim_block_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long, C.POINTER(C.c_float)]),
     "atsim_host_alloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t]),
     "atsim_host_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_copy_async": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),

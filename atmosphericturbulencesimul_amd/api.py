@@ -89,6 +89,16 @@ class Context:
         self._chk(self.L.atsim_block_fixup(self.h, dp(b), xs, dp(out)))
         return out
 
+    def block_counts(self, peaks):
+        """(xs, xs) float32 first-round counts of the fix-up for (xs, xs) block peaks"""
+        b = np.ascontiguousarray(peaks, np.float64)
+        xs = b.shape[0]
+        assert b.shape == (xs, xs)
+        out = np.zeros((xs, xs), np.float32)
+        self._chk(self.L.atsim_block_counts(self.h, b.ctypes.data_as(C.POINTER(C.c_double)), xs,
+                                            out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def set_tuning(self, key, value):
         self._chk(self.L.atsim_set_tuning(self.h, key.encode(), int(value)))
 

@@ -1190,6 +1190,35 @@ int atsim_block_fixup(atsim_ctx* c, const double* bsum, long xs, double* peaks) 
     return r;
 }
 
+int atsim_block_counts(atsim_ctx* c, const double* peaks, long xs, float* counts) {
+    if (!c || !peaks || !counts || xs <= 0 || xs > 4096) return ATSIM_EINVAL;
+    const size_t nb = (size_t)xs * xs;
+    double* d_bin = nullptr;
+    float* d_ch = nullptr;
+    unsigned* d_max = nullptr;
+    int* d_f = nullptr;
+    int r = ATSIM_OK;
+    if (hipMalloc(&d_bin, sizeof(double) * nb) != hipSuccess || hipMalloc(&d_ch, sizeof(float) * nb) != hipSuccess ||
+        hipMalloc(&d_max, sizeof(unsigned)) != hipSuccess || hipMalloc(&d_f, sizeof(int)) != hipSuccess)
+        r = fail(c, ATSIM_ENOMEM, "block counts: out of device memory");
+    const int f0 = 0;
+    const unsigned z0 = 0u;
+    if (r == ATSIM_OK && (copy_on_stream(c, d_bin, peaks, sizeof(double) * nb) != hipSuccess ||
+                          copy_on_stream(c, d_f, &f0, sizeof(int)) != hipSuccess ||
+                          copy_on_stream(c, d_max, &z0, sizeof(unsigned)) != hipSuccess))
+        r = fail(c, ATSIM_EHIP, "block counts: upload failed");
+    if (r == ATSIM_OK && launch_block_counts(d_bin, d_ch, d_max, d_f, 1, (int)xs, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "block counts: launch failed");
+    if (r == ATSIM_OK && copy_on_stream(c, counts, d_ch, sizeof(float) * nb) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "block counts: download failed");
+    (void)hipStreamSynchronize(c->stream);
+    hipFree(d_bin);
+    hipFree(d_ch);
+    hipFree(d_max);
+    hipFree(d_f);
+    return r;
+}
+
 int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {
     if (!s || !d_out) return ATSIM_EINVAL;
     atsim_ctx* c = s->ctx;

@@ -1415,27 +1415,33 @@ __constant__ int c_pairs[12][6] = {
     {0, 0, 2, 0, 2, 0}, {0, 2, 0, 0, 0, 2}, {1, 2, 0, 0, 1, 2}, {0, 2, 1, 0, 1, 2},
     {2, 2, 0, 0, 2, 2}, {0, 2, 2, 0, 2, 2}, {0, 1, 2, 0, 2, 1}, {2, 1, 0, 0, 2, 1}};
 
-// the fix-up count of block b: +-0.2 / +-0.15 for every neighbour relation
-// whose peaks differ by more than pi (AT.c:3667-3903)
-__device__ __forceinline__ double block_count(const double* B, int b, int xs) {
+// the fix-up count of block b as the reference's float image holds it
+// (WFprecision = 0, AT.c:3667-3903): +-0.2 / +-0.15 for every neighbour
+// relation whose peaks differ by more than pi.  The steps reach b relation by
+// relation and, within a relation, in the loop order (ii2 outer, jj2 inner)
+// of the pair's base block; each step is a double add rounded to float, so
+// the order shows in the last bits (a count that is 0.5 in exact arithmetic
+// can come out at 0.50000006 and pass plim = 0.5).
+__device__ __forceinline__ float block_count(const double* B, int b, int xs) {
     const int i2 = b % xs, j2 = b / xs;
     const double pb = B[b];
-    double c = 0.0;
+    float c = 0.f;
     for (int q = 0; q < 12; q++) {
         const int dx1 = c_pairs[q][0], dy1 = c_pairs[q][1], dx2 = c_pairs[q][2], dy2 = c_pairs[q][3];
         const int xm = c_pairs[q][4], ym = c_pairs[q][5];
         const double cq = q < 4 ? 0.2 : 0.15;
-        int x = i2 - dx1, y = j2 - dy1;  // b as the first block of the pair
-        if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
-            const double pv2 = B[(y + dy2) * xs + x + dx2];
-            if (pv2 > pb + M_PI) c += cq;
-            if (pb > pv2 + M_PI) c -= cq;
-        }
-        x = i2 - dx2; y = j2 - dy2;  // b as the second block
-        if (x >= 0 && y >= 0 && x < xs - xm && y < xs - ym) {
-            const double pv1 = B[(y + dy1) * xs + x + dx1];
-            if (pb > pv1 + M_PI) c -= cq;
-            if (pv1 > pb + M_PI) c += cq;
+        // b as the second block has the smaller base when the pair's offset
+        // (d2 - d1) is positive in x, or zero in x and positive in y
+        const int ox = dx2 - dx1, oy = dy2 - dy1;
+        const bool second_first = ox > 0 || (ox == 0 && oy > 0);
+        for (int t = 0; t < 2; t++) {
+            const bool as_first = (t == 0) ? !second_first : second_first;
+            const int x = i2 - (as_first ? dx1 : dx2), y = j2 - (as_first ? dy1 : dy2);
+            if (x < 0 || y < 0 || x >= xs - xm || y >= xs - ym) continue;
+            const double po = as_first ? B[(y + dy2) * xs + x + dx2] : B[(y + dy1) * xs + x + dx1];
+            const double pv1 = as_first ? pb : po, pv2 = as_first ? po : pb;
+            if (pv2 > pv1 + M_PI) c = (float)((double)c + (as_first ? cq : -cq));
+            if (pv1 > pv2 + M_PI) c = (float)((double)c + (as_first ? -cq : cq));
         }
     }
     return c;
@@ -1463,7 +1469,7 @@ __global__ void __launch_bounds__(256) k_block_counts(const double* __restrict__
     const int b = blockIdx.x * 256 + threadIdx.x;
     float a = 0.f;
     if (b < nb) {
-        const float c = (float)block_count(bin + (size_t)f * nb, b, xs);
+        const float c = block_count(bin + (size_t)f * nb, b, xs);
         reinterpret_cast<float*>(ch)[(size_t)f * nb + b] = c;
         a = fabsf(c);
     }
@@ -1507,9 +1513,9 @@ __global__ void __launch_bounds__(1024) k_block_fix(double* __restrict__ bin, do
         } else {
             double mx = 0.0;
             for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-                const double c = block_count(B, b, xs);
-                Cc[b] = (float)c;
-                mx = fmax(mx, fabs((double)(float)c));
+                const float c = block_count(B, b, xs);
+                Cc[b] = c;
+                mx = fmax(mx, (double)fabsf(c));
             }
             for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
             if (lane == 0) redd[wv] = mx;
@@ -1556,6 +1562,14 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cma
         attr = true;
     }
     k_block_fix<<<nfields, 1024, use_lds ? need : 0, st>>>(bin, ch, field_list, xs, use_lds, cmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_counts(const double* bin, float* ch, unsigned* cmax, const int* field_list, int nfields,
+                               int xs, hipStream_t st) {
+    if (nfields <= 0 || xs <= 0) return hipSuccess;
+    dim3 grid((xs * xs + 255) / 256, nfields);
+    k_block_counts<<<grid, 256, 0, st>>>(bin, reinterpret_cast<double*>(ch), field_list, xs, cmax);
     return hipGetLastError();
 }
 

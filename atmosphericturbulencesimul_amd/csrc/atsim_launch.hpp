@@ -65,6 +65,9 @@ hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
 // cmax: nfields unsigned of scratch (the first round's largest |count| per field)
 hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cmax, const int* field_list,
                             int nfields, int xs, hipStream_t st);
+// first-round counts alone (float, per field nb = xs^2), with cmax as above
+hipError_t launch_block_counts(const double* bin, float* ch, unsigned* cmax, const int* field_list, int nfields,
+                               int xs, hipStream_t st);
 hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,
                               const int* blk, int nfields, int Nout, int xs, hipStream_t st);
 

@@ -235,6 +235,12 @@ int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
  * (AtmosphericTurbulence.c:3627-4262), as atsim_sim_frame computes them.
  * xs <= 4096.  A test entry point with no counterpart in the reference. */
 int atsim_block_fixup(atsim_ctx *ctx, const double *bsum, long xs, double *peaks);
+/* First-round counts of the block fix-up for given block peaks [xs][xs]
+ * (host): counts [xs][xs] (host) as the reference's float image holds them
+ * with WFprecision = 0 (AtmosphericTurbulence.c:3667-3903: each +-0.2 /
+ * +-0.15 rounded to float in the reference's loop order).  xs <= 4096.  A
+ * test entry point with no counterpart in the reference. */
+int atsim_block_counts(atsim_ctx *ctx, const double *peaks, long xs, float *counts);
 
 /* Advance every layer by one time step without computing a frame (the
  * positions a following atsim_sim_layer_screens samples).  No counterpart in

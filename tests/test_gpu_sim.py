@@ -349,3 +349,23 @@ def test_block_fixup_against_the_oracle(ctx, oracle, case):
     peak_g = ctx.block_fixup(np.stack([bre, bim], -1))
     print(case, "oracle rounds", it, "max |peak diff|", np.abs(peak_g - peak_o).max())
     assert np.allclose(peak_g, peak_o, rtol=0, atol=1e-9)
+
+
+@pytest.mark.gpu
+def test_block_counts_float_order_against_the_oracle(ctx, oracle):
+    # first-round counts as the reference's float image accumulates them
+    # (WFprecision = 0): every +-0.2 / +-0.15 rounded to float in the
+    # reference's loop order; bit-exact against the oracle's float mode on
+    # unwrapped peak grids where blocks get steps of both signs
+    rng = np.random.default_rng(11)
+    differs = 0
+    for xs in (3, 7, 12, 25):
+        for _ in range(10):
+            peaks = rng.choice([-6.5, -3.3, 0.0, 3.3, 6.5], size=(xs, xs)).astype(np.float32).astype(np.float64)
+            g = ctx.block_counts(peaks)
+            _, ch32, _ = oracle.block_fix(peaks, prec_f32=True)
+            _, ch64, _ = oracle.block_fix(peaks)
+            assert np.array_equal(g.astype(np.float64), ch32), xs
+            differs += int(np.sum(ch32 != ch64.astype(np.float32).astype(np.float64)))
+    print("blocks whose float count differs from the rounded double count:", differs)
+    assert differs > 0
