@@ -75,6 +75,8 @@ _SIGS = {
                                               C.POINTER(C.c_double)]),
     "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atsim_sim_stats_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "atsim_sim_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long]),
     "atsim_sim_block_sums": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long]),
     "atsim_block_fixup": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long, C.POINTER(C.c_double)]),
     "atsim_block_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long, C.POINTER(C.c_float)]),

@@ -326,6 +326,17 @@ class Simulation:
         self.ctx._chk(self.ctx.L.atsim_sim_block_sums(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out
 
+    def stats_enable(self, enable=True):
+        """reduce every following frame's outputs to per-field statistics"""
+        self.ctx._chk(self.ctx.L.atsim_sim_stats_enable(self.h, int(enable)))
+
+    def stats(self):
+        """(D, W, 4) float64 of the last frame: OPD piston [m], OPD rms [m],
+        mean intensity, scintillation index"""
+        out = np.zeros((self.D, self.W, 4))
+        self.ctx._chk(self.ctx.L.atsim_sim_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
     def gaps(self):
         return self.ctx.L.atsim_sim_gaps(self.h)
 
@@ -351,7 +362,7 @@ class Simulation:
 
     # row / col: passes with one Fresnel factor; row2 / col2: passes with a
     # factor, a layer group's screen and a factor (one pass per gap)
-    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3, "row2": 4, "col2": 5, "phase": 6}
+    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3, "row2": 4, "col2": 5, "phase": 6, "stats": 7}
 
     def profile(self, enable=True):
         self.ctx._chk(self.ctx.L.atsim_sim_profile(self.h, int(enable)))

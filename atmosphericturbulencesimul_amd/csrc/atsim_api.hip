@@ -550,6 +550,11 @@ struct atsim_sim {
     int nslots = 0;
     int nblockfields = 0;
     int xs = 0;
+    // per-frame output statistics (atsim_sim_stats_enable): partial sums and
+    // the [F][4] sums of pha, pha^2, amp^2, amp^4 of the last frame
+    bool stats = false;
+    double* st_part = nullptr;
+    double* st_out = nullptr;
     // per-launch event timing (bench instrumentation)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -577,7 +582,8 @@ enum {
     PROF_ROW_GAP2 = 4,  // row pass: x factor, screen, x factor
     PROF_COL_GAP2 = 5,  // column pass: y factor, screen, y factor
     PROF_PHASE = 6,     // sampled phase planes of the column passes' layers
-    PROF_KINDS = 7
+    PROF_STATS = 7,     // per-frame output statistics
+    PROF_KINDS = 8
 };
 
 static hipEvent_t prof_event(atsim_sim* s) {
@@ -646,6 +652,8 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->flist_dev);
     hipFree(s->vq);
     hipFree(s->slot_layer_dev);
+    hipFree(s->st_part);
+    hipFree(s->st_out);
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     delete s;
 }
@@ -1024,6 +1032,11 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
                                      s->xs, c->stream));
         prof_end(s, pb);
     }
+    if (s->stats) {
+        long ps = prof_begin(s, PROF_STATS);
+        HIPCHK(c, launch_stats(d_pha, d_amp, F, s->Nout * s->Nout, s->st_part, s->st_out, c->stream));
+        prof_end(s, ps);
+    }
     return ATSIM_OK;
 }
 
@@ -1159,6 +1172,45 @@ int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
         return ATSIM_OK;
     }
     HIPCHK(c, copy_on_stream(c, out, s->bsum, sizeof(double) * need));
+    return ATSIM_OK;
+}
+
+int atsim_sim_stats_enable(atsim_sim* s, int enable) {
+    if (!s) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long F = s->D * s->W;
+    if (enable && !s->st_out) {
+        HIPCHK(c, hipMalloc(&s->st_part, sizeof(double) * F * STATS_PART_PER_FIELD));
+        HIPCHK(c, hipMalloc(&s->st_out, sizeof(double) * F * 4));
+        HIPCHK(c, hipMemsetAsync(s->st_out, 0, sizeof(double) * F * 4, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if ((enable != 0) != s->stats) sim_drop_graph(s);  // the captured frames change
+    s->stats = enable != 0;
+    return ATSIM_OK;
+}
+
+int atsim_sim_stats(atsim_sim* s, double* out, long n) {
+    if (!s || !out) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    const long F = s->D * s->W;
+    if (n < F * 4) return fail(c, ATSIM_EINVAL, "sim_stats: need %ld values, got %ld", F * 4, n);
+    if (!s->st_out) return fail(c, ATSIM_EINVAL, "sim_stats: statistics are not enabled");
+    std::vector<double> sums((size_t)F * 4);
+    HIPCHK(c, hipMemcpyAsync(sums.data(), s->st_out, sizeof(double) * F * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double np = (double)s->Nout * (double)s->Nout;
+    for (long f = 0; f < F; f++) {
+        const double lam = s->lambda[f % s->W];
+        const double mp = sums[f * 4] / np;
+        const double vp = std::max(sums[f * 4 + 1] / np - mp * mp, 0.0);
+        const double mi = sums[f * 4 + 2] / np;
+        const double mi2 = sums[f * 4 + 3] / np;
+        out[f * 4 + 0] = mp * lam / (2.0 * M_PI);
+        out[f * 4 + 1] = sqrt(vp) * lam / (2.0 * M_PI);
+        out[f * 4 + 2] = mi;
+        out[f * 4 + 3] = mi > 0.0 ? mi2 / (mi * mi) - 1.0 : 0.0;
+    }
     return ATSIM_OK;
 }
 

@@ -1644,6 +1644,97 @@ hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, flo
 // ------------------------------------------------------------------ fill
 // zero n doubles (a kernel, not a memset, so that it can sit in a captured
 // frame graph)
+// ------------------------------------------------------------------ frame statistics
+// Per field of one frame: sums of the output phase, its square, the intensity
+// I = amp^2 and I^2 over the Nout^2 output pixels, accumulated in FP64.  The
+// first kernel reduces a contiguous slice of one field per workgroup (wave
+// reduction by DPP-free xor shuffles, then across the 4 waves in LDS) to
+// part[f][g][4]; the second adds a field's STAT_G slices in a fixed order, so
+// the sums do not depend on scheduling.
+constexpr int STAT_G = 128;  // slices per field
+constexpr int STAT_T = 256;
+
+__device__ __forceinline__ double wave_sum_f64(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__global__ void __launch_bounds__(STAT_T) k_stats_partial(const float* __restrict__ pha, const float* __restrict__ amp,
+                                                          long n, double* __restrict__ part) {
+    const int f = blockIdx.y, g = blockIdx.x;
+    // slices of whole float4 groups, so every slice starts 16-byte aligned
+    const long n4 = n >> 2;
+    const long per = (n4 + STAT_G - 1) / STAT_G;
+    const long q0 = min(n4, g * per), q1 = min(n4, q0 + per);
+    const float* P = pha + (size_t)f * n;
+    const float* A = amp + (size_t)f * n;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+    const bool vec = ((n & 3) == 0) && ((reinterpret_cast<uintptr_t>(P) | reinterpret_cast<uintptr_t>(A)) & 15) == 0;
+    if (vec) {
+        for (long q = q0 + threadIdx.x; q < q1; q += STAT_T) {
+            const float4 p = reinterpret_cast<const float4*>(P)[q];
+            const float4 a = reinterpret_cast<const float4*>(A)[q];
+            const double pv[4] = {p.x, p.y, p.z, p.w};
+            const double av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double I = av[k] * av[k];
+                s1 += pv[k];
+                s2 = fma(pv[k], pv[k], s2);
+                s3 += I;
+                s4 = fma(I, I, s4);
+            }
+        }
+    } else {
+        // scalar pixels of the same slices; the last slice also takes the n % 4 tail
+        const long e0 = q0 * 4, e1 = (g == STAT_G - 1) ? n : q1 * 4;
+        for (long e = e0 + threadIdx.x; e < e1; e += STAT_T) {
+            const double pv = P[e], av = A[e];
+            const double I = av * av;
+            s1 += pv;
+            s2 = fma(pv, pv, s2);
+            s3 += I;
+            s4 = fma(I, I, s4);
+        }
+    }
+    __shared__ double red[STAT_T / 64][4];
+    s1 = wave_sum_f64(s1);
+    s2 = wave_sum_f64(s2);
+    s3 = wave_sum_f64(s3);
+    s4 = wave_sum_f64(s4);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = s1;
+        red[wv][1] = s2;
+        red[wv][2] = s3;
+        red[wv][3] = s4;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int k = 0; k < STAT_T / 64; k++) t += red[k][threadIdx.x];
+        part[((size_t)f * STAT_G + g) * 4 + threadIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_stats_final(const double* __restrict__ part, double* __restrict__ out) {
+    const int f = blockIdx.x;
+    // lane = 4 g' + k: lane sums value k of slices g', g' + 16, ...
+    const int k = threadIdx.x & 3, g0 = threadIdx.x >> 2;
+    double t = 0.0;
+    for (int g = g0; g < STAT_G; g += 16) t += part[((size_t)f * STAT_G + g) * 4 + k];
+    for (int o = 32; o >= 4; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (threadIdx.x < 4) out[(size_t)f * 4 + k] = t;
+}
+
+hipError_t launch_stats(const float* pha, const float* amp, long F, long n, double* part, double* out,
+                        hipStream_t st) {
+    if (F <= 0 || n <= 0) return hipSuccess;
+    k_stats_partial<<<dim3(STAT_G, (unsigned)F), STAT_T, 0, st>>>(pha, amp, n, part);
+    k_stats_final<<<(unsigned)F, 64, 0, st>>>(part, out);
+    return hipGetLastError();
+}
+
 __global__ void k_zero_f64(double* __restrict__ p, long n) {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0.0;
 }

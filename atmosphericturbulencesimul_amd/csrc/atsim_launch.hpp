@@ -72,6 +72,11 @@ hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, 
                               const int* blk, int nfields, int Nout, int xs, hipStream_t st);
 
 hipError_t launch_zero_f64(double* p, long n, hipStream_t st);
+// per-field sums of pha, pha^2, amp^2, amp^4 over n pixels of F fields:
+// out[F][4]; part holds F * 128 * 4 doubles of partial sums
+constexpr long STATS_PART_PER_FIELD = 128 * 4;
+hipError_t launch_stats(const float* pha, const float* amp, long F, long n, double* part, double* out,
+                        hipStream_t st);
 hipError_t launch_layer_screens(const float* masters, const LayerSample* ls, float* out, long N, long M, long L,
                                 long D, hipStream_t st);
 

@@ -215,8 +215,9 @@ long atsim_sim_gaps(atsim_sim *sim);
  * with one factor (screen + x factor, or x factor alone), 1 column pass with
  * one factor, 2 extraction, 3 block unwrap, 4 row pass with two x factors
  * and a screen, 5 column pass with two y factors and a screen, 6 sampling of
- * the phase planes that column passes apply; returns the summed duration
- * [ms] and the number of launches recorded since the last enable. */
+ * the phase planes that column passes apply, 7 output statistics; returns
+ * the summed duration [ms] and the number of launches recorded since the
+ * last enable. */
 int atsim_sim_profile(atsim_sim *sim, int enable);
 int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
 
@@ -228,6 +229,16 @@ int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
  * the context stream.  A diagnostic for tests, with no entry point in the
  * reference. */
 int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
+
+/* Output statistics of every frame (the OPD / scintillation statistics
+ * stream; no counterpart in the reference, which leaves them to
+ * post-processing).  With enable = 1 every following frame also reduces its
+ * outputs per field, in FP64 over the Nout^2 pixels; 0 stops it (the default).
+ * atsim_sim_stats writes, for the last frame, out [D][W][4] (host, n >= 4 D W):
+ * OPD piston <pha> lambda / 2 pi [m], OPD rms [m], mean intensity <amp^2> and
+ * scintillation index <I^2> / <I>^2 - 1.  Synchronises the context stream. */
+int atsim_sim_stats_enable(atsim_sim *sim, int enable);
+int atsim_sim_stats(atsim_sim *sim, double *out, long n);
 
 /* The block fix-up of the science-lambda rule on given block sums: bsum
  * [xs][xs][2] (host; sum cos, sum sin per block), peaks [xs][xs] (host) the

@@ -170,6 +170,36 @@ def test_headline_energy_and_sampled_phase(ctx, oracle):
     sim2.close()
 
 
+def test_headline_against_the_oracle(ctx, oracle):
+    # the headline workload itself: 2048^2, 10 layers (9 Fresnel gaps), 8
+    # wavelengths 0.5-2.2 um with the reference rule at 0.5 um and the block
+    # rule elsewhere, bicubic sampling, 8192^2 masters synthesised and
+    # normalised on the GPU and handed to the FP64 oracle; two frames, every
+    # field against the oracle at the bar of test_gpu_sim
+    print("case: 2048^2, M=8192, 10 layers, 8 wavelengths, 2 frames against the oracle", flush=True)
+    N, M, dx, L = 2048, 8192, 0.01, 10
+    alts = np.linspace(0.0, 20000.0, L)
+    cn2 = np.exp(-alts / 2000.0)
+    speeds = np.linspace(5.0, 40.0, L)
+    dirs = np.arange(L) * 2.399963229728653
+    lams = np.linspace(0.5e-6, 2.2e-6, 8)
+    sim, p = _gpu_masters_case(ctx, N, M, alts, cn2, speeds, dirs, lams, 0.5e-6, dx, 0.12, 20.0, 0.01,
+                               seed=0xA75 + 4000)
+    assert sim.gaps() == 9
+    masters = np.stack([sim.master(l).cpu().numpy() for l in range(L)]).astype(np.float64)
+    osim = oracle.Sim(N, N, masters, p["sa"], p["sli"], p["lambdas"], p["scoeff"], p["unwrap"], p["x0"][0],
+                      p["y0"][0], p["vx"], p["vy"], dx, 0.001, 0.0, True)
+    del masters
+
+    class C_:
+        pass
+    c = C_()
+    c.D, c.W, c.N, c.Nout = 1, 8, N, N
+    worst = _compare(c, [osim], sim, nframes=2)
+    print("headline worst against the FP64 oracle:", worst, flush=True)
+    sim.close()
+
+
 def test_mixed_sizes_sixteen_directions(ctx, oracle):
     # 512^2, 1024^2 and 2048^2 simulations (one per grid size) over the same
     # 10 masters of 8192^2, 8 wavelengths and 16 directions on a 4 x 4 grid

@@ -113,8 +113,12 @@ def test_noise_matches_oracle(ctx, oracle):
     assert np.mean(g == o32) > 0.999
 
 
-@pytest.mark.parametrize("size,L0,l0", [(256, 1000.0, 1.0), (1024, 2000.0, 0.5)])
+@pytest.mark.parametrize("size,L0,l0", [(256, 1000.0, 1.0), (1024, 2000.0, 0.5), (4096, 2000.0, 1.0),
+                                        (8192, 2000.0, 1.0)])
 def test_make_master_vs_oracle(ctx, oracle, size, L0, l0):
+    # 8192^2 is the master size of the 2048^2 headline workload (L0 = 20 m and
+    # l0 = 1 cm at 1 cm pixels)
+    print(f"make_master {size}^2 L0 {L0} px l0 {l0} px", flush=True)
     z = ctx.noise(size * size, 0xA75 + size, 0).reshape(size, size)
     zn = z.cpu().numpy().astype(np.complex128)
     o1, o2, c1, c2 = oracle.make_master(zn, L0, l0)
