@@ -101,6 +101,30 @@ static void fresnel_table(float2* h, long n, double dx, double z, double lambda)
     }
 }
 
+// ------------------------------------------------------------------ allocation-free Fresnel
+// Device memory of atsim_fresnel_noalloc: the 1D transfer factor h(k) and the
+// twiddles e^{-2 pi i m / n} of one size, n <= ATSIM_NOALLOC_MAX_N (128 KB per
+// device, held until the process ends).  Every call on a device shares its
+// copy, and calls that overlap on one device race on it.
+#define ATSIM_NOALLOC_MAX_N 8192
+#define ATSIM_MAX_DEVICES 64
+static __device__ __align__(256) float2 scratch[2 * ATSIM_NOALLOC_MAX_N];
+// device address of `scratch` per device, from hipGetSymbolAddress at the
+// first context on that device (the host-side name is only a stand-in)
+static float2* g_scratch[ATSIM_MAX_DEVICES];
+
+// tab[k] = h(k) as fresnel_table builds it, tw[m] = e^{-2 pi i m / n} as
+// get_twiddles builds it, both in FP64 and rounded to float
+__global__ void k_fresnel_tables(float2* __restrict__ tab, float2* __restrict__ tw, long n, double coeff) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double kt = (double)(k < n / 2 ? k : k - n);
+    const double ang = -coeff * kt * kt;
+    tab[k] = make_float2((float)(cos(ang) / n), (float)(sin(ang) / n));
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
+}
+
 extern "C" {
 
 int atsim_ctx_create(int device, atsim_ctx** out) {
@@ -110,6 +134,11 @@ int atsim_ctx_create(int device, atsim_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ATSIM_EHIP;
     if (device < 0 || device >= ndev) return ATSIM_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return ATSIM_EHIP;
+    if (device < ATSIM_MAX_DEVICES && !g_scratch[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return ATSIM_EHIP;
+        g_scratch[device] = (float2*)p;
+    }
     atsim_ctx* c = new atsim_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -325,6 +354,32 @@ int atsim_fresnel(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbat
     hipFreeAsync(d_tab, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host table must outlive the copy
     return r;
+}
+
+int atsim_fresnel_noalloc(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbatch, double dx, double z,
+                          double lambda) {
+    if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    if (n > ATSIM_NOALLOC_MAX_N)
+        return fail(c, ATSIM_EUNSUPPORTED, "fresnel_noalloc: size %ld exceeds the maximum %d", n, ATSIM_NOALLOC_MAX_N);
+    if (!supported_n(n))
+        return fail(c, ATSIM_EUNSUPPORTED, "fresnel_noalloc: size %ld is not a power of two in [16, 8192]", n);
+    if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fresnel_noalloc: nbatch %ld out of range", nbatch);
+    if (!(dx > 0.0)) return fail(c, ATSIM_EINVAL, "fresnel_noalloc: pupil scale must be > 0");
+    float2* base = c->device < ATSIM_MAX_DEVICES ? g_scratch[c->device] : nullptr;
+    if (!base) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_noalloc: no scratch memory on device %d", c->device);
+    float2* d_tab = base;
+    float2* d_tw = base + ATSIM_NOALLOC_MAX_N;
+    HIPCHK(c, hipSetDevice(c->device));
+    const double coeff = M_PI * z * lambda / (dx * n) / (dx * n);
+    k_fresnel_tables<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d_tab, d_tw, n, coeff);
+    HIPCHK(c, hipGetLastError());
+    if (d_in != d_out)
+        HIPCHK(c, hipMemcpyAsync(d_out, d_in, sizeof(float2) * n * n * nbatch, hipMemcpyDeviceToDevice, c->stream));
+    ScreenArgs none{};
+    none.W = 1;
+    HIPCHK(c, launch_row(n, OP_FRESNEL, 0, false, (float2*)d_out, nbatch, d_tab, d_tw, none, c->stream));
+    HIPCHK(c, launch_col(n, OP_FRESNEL, 0, (float2*)d_out, nbatch, d_tab, 1, d_tw, c->stream));
+    return ATSIM_OK;
 }
 
 int atsim_fresnel_tf_init(atsim_ctx* c, void* d_C, long n, double dx, double z, double lambda, double fpmaskrad) {

@@ -108,6 +108,15 @@ int atsim_permut(atsim_ctx *ctx, void *d_data, long n0, long n1, long nbatch);
  * fields.  d_in may equal d_out. */
 int atsim_fresnel(atsim_ctx *ctx, const void *d_in, void *d_out, long n, long nbatch,
                   double pupil_scale, double z, double lambda);
+/* atsim_fresnel without allocating: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The transfer factor
+ * and twiddle tables go to a fixed 128 KB device array of the library
+ * (n <= 8192; larger sizes return ATSIM_EUNSUPPORTED) that every call on a
+ * device shares: calls that overlap on one device race.  Asynchronous on the
+ * context stream (atsim_sync waits); same results as atsim_fresnel. */
+int atsim_fresnel_noalloc(atsim_ctx *ctx, const void *d_in, void *d_out, long n, long nbatch,
+                          double pupil_scale, double z, double lambda);
 /* Init_Fresnel_propagate_wavefront (WFpropagate.c:166-261): fills the
  * anti-aliased, permuted transfer function C (n x n complex, device). */
 int atsim_fresnel_tf_init(atsim_ctx *ctx, void *d_C, long n, double pupil_scale, double z,
