@@ -326,6 +326,42 @@ class Simulation:
         self.ctx._chk(self.ctx.L.atsim_sim_block_sums(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out
 
+    def autotune(self, frames=5, cache=None):
+        """Pick the context's launch shapes for this simulation with the
+        library's autotuner (atsim_sim_autotune) and return the winning
+        settings.  cache: path of a JSON file of winners keyed by device and
+        sizes (default $ATSIM_TUNING_CACHE, no file if unset); a cached entry
+        is applied without a search."""
+        import json
+        import os
+        torch = _torch()
+        path = cache if cache is not None else os.environ.get("ATSIM_TUNING_CACHE")
+        key = "|".join(str(v) for v in (torch.cuda.get_device_name(self.ctx.device), self.N, self.Nout, self.M,
+                                        self.L, self.W, self.D))
+        table = {}
+        if path and os.path.exists(path):
+            with open(path) as fh:
+                table = json.load(fh)
+        if key in table:
+            for k, v in table[key].items():
+                self.ctx.set_tuning(k, v)
+            return dict(table[key], cached=True)
+        buf = C.create_string_buffer(1 << 16)
+        self.ctx._chk(self.ctx.L.atsim_sim_autotune(self.h, _ptr(self.pha), _ptr(self.amp), int(frames), buf,
+                                                    len(buf)))
+        lines = buf.value.decode().strip().splitlines()
+        best = dict(kv.split("=") for kv in lines[-1].split()[1:])
+        ms = float(best.pop("ms"))
+        best = {k: int(v) for k, v in best.items()}
+        if path:
+            table[key] = best
+            d = os.path.dirname(os.path.abspath(path))
+            os.makedirs(d, exist_ok=True)
+            with open(path, "w") as fh:
+                json.dump(table, fh, indent=1)
+        self.tune_report = lines
+        return dict(best, ms_per_frame=ms, candidates=len(lines) - 1, cached=False)
+
     def stats_enable(self, enable=True):
         """reduce every following frame's outputs to per-field statistics"""
         self.ctx._chk(self.ctx.L.atsim_sim_stats_enable(self.h, int(enable)))

@@ -1188,6 +1188,130 @@ int atsim_sim_normalize_masters(atsim_sim* s, double r0_goal_px, const double* c
     return ATSIM_OK;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Launch-shape search for one simulation: coordinate descent over the tuning
+// keys (two sweeps), each candidate timed over `frames` frames with events on
+// the context stream after two untimed frames (the first runs eagerly after
+// the change, the second captures the frame graph when graphs are on).
+namespace {
+struct TuneCfg {
+    int row_mw, col_c, col_e, col2_e, streams, gap_passes;
+};
+TuneCfg tune_get(const atsim_ctx* c) { return {c->row_mw, c->col_c, c->col_e, c->col2_e, c->streams, c->gap_passes}; }
+void tune_set(atsim_ctx* c, const TuneCfg& t) {
+    c->row_mw = t.row_mw;
+    c->col_c = t.col_c;
+    c->col_e = t.col_e;
+    c->col2_e = t.col2_e;
+    c->streams = t.streams;
+    c->gap_passes = t.gap_passes;
+    c->tuning_gen++;
+}
+int& tune_field(TuneCfg& t, int k) {
+    switch (k) {
+        case 0: return t.gap_passes;
+        case 1: return t.streams;
+        case 2: return t.row_mw;
+        case 3: return t.col2_e;
+        case 4: return t.col_c;
+        default: return t.col_e;
+    }
+}
+const char* tune_name(int k) {
+    static const char* names[] = {"gap_passes", "streams", "row_mw", "col2_e", "col_c", "col_e"};
+    return names[k];
+}
+}  // namespace
+
+int atsim_sim_autotune(atsim_sim* s, float* d_pha, float* d_amp, int frames, char* report, long report_len) {
+    if (!s || !d_pha || !d_amp || frames <= 0 || frames > 1000) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    if (s->prof) return fail(c, ATSIM_EINVAL, "autotune: per-launch profiling is on");
+    const long N = s->N, W = s->W;
+    std::vector<double> xp((size_t)(s->D * s->L)), yp((size_t)(s->D * s->L));
+    xp.assign(s->xpos.begin(), s->xpos.end());
+    yp.assign(s->ypos.begin(), s->ypos.end());
+    // candidate values per key (only shapes the kernels support at this N)
+    std::vector<int> cand[6];
+    cand[0] = {1, 2};
+    for (int v = 1; v <= 4 && v <= W; v++) cand[1].push_back(v);
+    const long mwmax = std::min<long>(8, std::max<long>(1, 16384 / N));
+    for (int v = 1; v <= mwmax; v *= 2) cand[2].push_back(v);
+    cand[3] = (N >= 256 && N <= 2048) ? std::vector<int>{16, 32} : std::vector<int>{16};
+    for (int v : {2, 4, 8, 16})
+        if ((size_t)v * (N + N / 16) * 8 <= 160 * 1024 && v <= N) cand[4].push_back(v);
+    if (cand[4].empty()) cand[4].push_back(0);
+    cand[5] = (N >= 256 && N <= 2048) ? std::vector<int>{16, 32} : std::vector<int>{16};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return fail(c, ATSIM_EHIP, "autotune: event creation failed");
+    }
+    const TuneCfg start = tune_get(c);
+    std::string rep;
+    int r = ATSIM_OK;
+    auto measure = [&](const TuneCfg& t, double* ms) -> int {
+        tune_set(c, t);
+        for (int k = 0; k < 2; k++)
+            if (int rr = atsim_sim_frame(s, d_pha, d_amp, nullptr)) return rr;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(c, ATSIM_EHIP, "autotune: event record");
+        for (int k = 0; k < frames; k++)
+            if (int rr = atsim_sim_frame(s, d_pha, d_amp, nullptr)) return rr;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(c, ATSIM_EHIP, "autotune: event wait");
+        float el = 0.f;
+        if (hipEventElapsedTime(&el, e0, e1) != hipSuccess) return fail(c, ATSIM_EHIP, "autotune: elapsed");
+        *ms = el / frames;
+        char line[160];
+        snprintf(line, sizeof line, "gap_passes=%d streams=%d row_mw=%d col2_e=%d col_c=%d col_e=%d ms=%.4f\n",
+                 t.gap_passes, t.streams, t.row_mw, t.col2_e, t.col_c, t.col_e, *ms);
+        rep += line;
+        return ATSIM_OK;
+    };
+    TuneCfg best = start;
+    double best_ms = 0.0;
+    r = measure(best, &best_ms);
+    for (int sweep = 0; sweep < 2 && r == ATSIM_OK; sweep++) {
+        bool changed = false;
+        for (int k = 0; k < 6 && r == ATSIM_OK; k++) {
+            for (int v : cand[k]) {
+                TuneCfg t = best;
+                if (tune_field(t, k) == v) continue;
+                tune_field(t, k) = v;
+                double ms = 0.0;
+                r = measure(t, &ms);
+                if (r) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                    changed = true;
+                }
+            }
+        }
+        if (!changed) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    tune_set(c, r == ATSIM_OK ? best : start);
+    // the simulation continues from where it was before the search
+    for (size_t i = 0; i < xp.size(); i++) {
+        s->xpos[i] = xp[i];
+        s->ypos[i] = yp[i];
+    }
+    if (r == ATSIM_OK) {
+        char line[200];
+        snprintf(line, sizeof line, "best gap_passes=%d streams=%d row_mw=%d col2_e=%d col_c=%d col_e=%d ms=%.4f\n",
+                 best.gap_passes, best.streams, best.row_mw, best.col2_e, best.col_c, best.col_e, best_ms);
+        rep += line;
+    }
+    if (report && report_len > 0) {
+        strncpy(report, rep.c_str(), (size_t)report_len - 1);
+        report[report_len - 1] = '\0';
+    }
+    return r;
+}
+
 int atsim_sim_profile(atsim_sim* s, int enable) {
     if (!s) return ATSIM_EINVAL;
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));

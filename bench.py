@@ -106,24 +106,40 @@ def read_valu():
     return read_pass_table("valu_headline_2048_10L_8lam.md", "wave_instructions_per_launch")
 
 
-def cpu_baseline_and_error(ctx, wl, masters_gpu, frames=2, threads=0):
-    """Oracle (FP64) on a bounded sample of the same workload: the 2048^2,
-    10-layer, bicubic, Fresnel layer loop for all 8 wavelengths (reference rule
-    at 0.5 um, block rule elsewhere) with 2048^2 masters cut from the GPU's,
-    one oracle simulation per wavelength on its own host thread.  The GPU runs
-    the same sample for the FP64 error of every field."""
+def _fftw_backend():
+    """Name of the FFT library a reference build on this host would use: FFTW
+    (libfftw3f.so.3, src/fft/fft.c:52) if the host has it, else none"""
+    import ctypes
+    for name in ("libfftw3f.so.3", "libfftw3f.so"):
+        try:
+            ctypes.CDLL(name)
+            return name
+        except OSError:
+            pass
+    return None
+
+
+def _oracle_case(wl, masters_gpu, M):
+    """host float32 masters cut to M^2 from the GPU's, and the simulation set-up"""
+    ms = np.stack([m[:M, :M].cpu().numpy() for m in masters_gpu]).astype(np.float32)
+    return ms
+
+
+def fp64_error(ctx, wl, ms, frames=1, threads=0):
+    """FP64 error of every full-size field: the GPU and the FP64 oracle run the
+    same frames of the 2048^2, 10-layer, bicubic, Fresnel loop for all 8
+    wavelengths (reference rule at 0.5 um, block rule elsewhere) over masters
+    cut to 2048^2 from the GPU's; one oracle simulation per wavelength, each on
+    its own host thread"""
     import torch
     from concurrent.futures import ThreadPoolExecutor
     from oracle import pyoracle as po
-    M = wl["N"]
-    L = wl["L"]
-    W = wl["W"]
-    ms = np.stack([m[:M, :M].cpu().numpy() for m in masters_gpu]).astype(np.float32)
+    M = ms.shape[-1]
+    L, W = wl["L"], wl["W"]
     sim, p = make_sim(ctx, wl, M=M)
     for l in range(L):
         sim.set_master(l, ms[l])
     ms64 = ms.astype(np.float64)
-    del ms
     osims = [po.Sim(wl["N"], wl["Nout"], ms64, p["sa"], p["sli"], [wl["lambdas"][w]], p["scoeff"][w:w + 1],
                     [int(p["unwrap"][w])], p["x0"][0], p["y0"][0], p["vx"], p["vy"], wl["dx"], wl["dt"], 0.0, True)
               for w in range(W)]
@@ -134,12 +150,16 @@ def cpu_baseline_and_error(ctx, wl, masters_gpu, frames=2, threads=0):
         gpu.append((field.cpu().numpy()[0], pha_g.cpu().numpy()[0], amp_g.cpu().numpy()[0]))
     sim.close()
     threads = threads or min(W, 16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
         outs = list(ex.map(lambda o: [o.frame() for _ in range(frames)], osims))
-    t_cpu = time.perf_counter() - t0
-    errs = dict(field=0.0, amp=0.0, opd=0.0, unwrap_mismatch=0)
+    del ms64
+    errs = dict(field=0.0, amp=0.0, opd=0.0, unwrap_mismatch=0, scint_index=0.0)
     rl = lambda a, b: float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    def si(a):
+        I = a.astype(np.float64) ** 2
+        return float(I.var() / I.mean() ** 2)
+
     for w in range(W):
         for k in range(frames):
             E_o, _, pha_o, amp_o = outs[w][k]
@@ -149,19 +169,81 @@ def cpu_baseline_and_error(ctx, wl, masters_gpu, frames=2, threads=0):
             bad = np.abs(pg - pha_o[0]) > 1.0
             errs["unwrap_mismatch"] = max(errs["unwrap_mismatch"], int(bad.sum()))
             errs["opd"] = max(errs["opd"], rl(pg[~bad], pha_o[0][~bad]))
-    cpu = {
-        "value": frames * W / t_cpu,
+            s_o = si(amp_o[0])
+            errs["scint_index"] = max(errs["scint_index"], abs(si(ag) - s_o) / s_o)
+    errs["case"] = (f"all {W} fields (0.5-2.2 um), 2048^2, 10 layers, {frames} frame(s), vs FP64 oracle (max "
+                    "over fields and frames; OPD over pixels whose 2 pi decision agrees; scintillation index "
+                    "sigma_I^2 relative error)")
+    return errs
+
+
+def cpu_baseline(wl, ms, frames=1, threads=0):
+    """The reference's own structure on the host cores: one run per science
+    wavelength, each carrying the reference-wavelength field beside its own
+    (AtmosphericTurbulence.c:2672, 2685), float storage with FP64 per-pixel
+    arithmetic (WFprecision = 0, AT.c:1472-1615), bicubic sampling, block rule
+    for the science field.  The 8 distinct wavefront-frames of a step (7
+    science wavelengths + the reference one) take the 7 two-field runs.
+    Timed on one core (the reference's path is single-threaded) and with the
+    runs spread over the host cores."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import pyoracle as po
+    from atmosphericturbulencesimul_amd.api import superlayers
+    L, W = wl["L"], wl["W"]
+    M = ms.shape[-1]
+    sa, sli = superlayers(wl["alts"], wl["cn2"], wl["bin_m"])
+    lam_ref = wl["lam_ref"]
+    sci = [lam for lam in wl["lambdas"] if abs(lam - lam_ref) > 1e-12]
+    x0 = np.full(L, 0.05 * M)
+    vx = wl["speeds"] * np.cos(wl["dirs"]) / wl["dx"]
+    vy = wl["speeds"] * np.sin(wl["dirs"]) / wl["dx"]
+    ms64 = ms.astype(np.float64)
+
+    def run(lam):
+        return po.Sim(wl["N"], wl["Nout"], ms64, sa, sli, [lam_ref, lam],
+                      np.array([[1.0] * L, [lam_ref / lam] * L]), [0, 1], x0, x0, vx, vy, wl["dx"], wl["dt"],
+                      0.0, True, prec_f32=True)
+
+    # one core: one two-field run
+    r1 = run(sci[-1])
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        r1.frame(want_field=False)
+    t_one = (time.perf_counter() - t0) / frames  # seconds per two-field run frame
+    del r1
+    # all cores: the 7 runs side by side
+    threads = threads or min(len(sci), 16, os.cpu_count() or 1)
+    runs = [run(lam) for lam in sci]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda r: [r.frame(want_field=False) for _ in range(frames)], runs))
+    t_all = time.perf_counter() - t0
+    del runs, ms64
+    fftw = _fftw_backend()
+    cpu_model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                cpu_model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": round(frames * W / t_all, 4),
         "unit": "wavefront-frames/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{frames} frames x {W} wavelengths of the 2048^2, 10-layer, 9-gap bicubic layer loop with "
-                  f"both unwrap rules (FP64 oracle, own radix-2 FFT, one wavelength per thread; masters "
-                  f"2048^2 cut from the GPU's), {t_cpu:.1f} s on {threads} threads of "
-                  f"{platform.processor() or platform.machine()}",
+        "single_core": round(W / (len(sci) * t_one), 4),
+        "fft_backend": "oracle radix-2 FFT (FP64 arithmetic, fields stored as float)" +
+                       ("" if fftw is None else f"; {fftw} present but not used"),
+        "fftw_present": fftw is not None,
+        "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model},
+        "sample": f"{frames} frame(s) of the 2048^2, 10-layer, 9-gap bicubic workload as the reference runs it: "
+                  f"{len(sci)} runs (one per science wavelength), each propagating the 0.5 um reference field "
+                  f"beside its own, float storage with FP64 per-pixel arithmetic (WFprecision = 0), masters "
+                  f"{M}^2 cut from the GPU's; {W} distinct wavefront-frames per step; all cores: the runs on "
+                  f"{threads} threads, {t_all:.1f} s; one core: one run, {t_one:.1f} s per frame",
     }
-    errs["case"] = (f"all {W} fields (0.5-2.2 um), 2048^2, 10 layers, {frames} frames, vs FP64 oracle "
-                    "(max over fields and frames; OPD over pixels whose 2 pi decision agrees)")
-    return cpu, errs
 
 
 def main():
@@ -170,7 +252,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / FP64 error sample")
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="host threads of the CPU baseline (0 = one per "
                     "wavelength, at most 16)")
     ap.add_argument("--row-mw", type=int, default=0, help="wavelengths per row-pass workgroup (0 = default)")
@@ -183,6 +265,8 @@ def main():
                     "a screen (0 = default, 16, 32)")
     ap.add_argument("--gap-passes", type=int, default=0, help="passes over the fields per Fresnel gap (0 = "
                     "default, 1: x / y order alternating; 2: a row and a column pass)")
+    ap.add_argument("--autotune", type=int, default=1, help="1: launch shapes picked by the library's autotuner "
+                    "(atsim_sim_autotune) before the timed frames; 0: built-in defaults or the flags above")
     ap.add_argument("--graph", type=int, default=1, help="1: frames replay a captured HIP graph of their kernels "
                     "(0: eager launches)")
     ap.add_argument("--host-copy-steps", type=int, default=20,
@@ -229,6 +313,11 @@ def main():
     sim.synth_masters(wl["seed"], wl["L0"] / wl["dx"], wl["l0"] / wl["dx"])
     r0_meas = sim.normalize_masters(wl["r0"] / wl["dx"], wl["cn2"])
     t_setup = time.perf_counter() - t0
+    tune = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune = sim.autotune(frames=5)
+        tune["search_s"] = round(time.perf_counter() - t0, 3)
     G = sim.gaps()
     F = wl["W"] * wl["D"]
 
@@ -369,19 +458,26 @@ def main():
     # whole-frame algorithmic bytes per wavefront-frame: the passes above,
     # the extraction's c64 read of the field, an f32 OPD and amplitude write
     frame_bytes = (sum(b for _, b in passes) + 8.0 * N * N * F) / F + 8.0 * wl["Nout"] ** 2
-    kernel_names = {"row": "k_row_fresnel<2048> (x pass: screen phase, FFT, transfer function, IFFT)",
-                    "col": "k_col<2048> (y pass: FFT, transfer function, IFFT)",
-                    "row2": "k_row_fresnel<2048, PRE> (x pass: FFT, transfer function, IFFT, screen phase, "
-                            "FFT, transfer function, IFFT)",
-                    "col2": "k_col_screen_split<2048, 8, 32> (y pass: FFT, transfer function, IFFT, screen "
-                            "phase, FFT, transfer function, IFFT)"}
+    c2e = ctx.get_tuning("col2_e") or (32 if 256 <= N <= 2048 else 16)
+    ce = ctx.get_tuning("col_e") or 16
+    cc = ctx.get_tuning("col_c") or (8 if N == 2048 else 16)
+    kernel_names = {"row": f"k_row_fresnel<{N}> (x pass: screen phase, FFT, transfer function, IFFT)",
+                    "col": (f"k_col_split<{N}, 8, 32>" if ce == 32 and 256 <= N <= 2048 and cc in (8, 16)
+                            else f"k_col<{N}, 1, 0, {cc}>") + " (y pass: FFT, transfer function, IFFT)",
+                    "row2": f"k_row_fresnel<{N}, false, true> (x pass: FFT, transfer function, IFFT, screen "
+                            "phase, FFT, transfer function, IFFT)",
+                    "col2": (f"k_col_screen_split<{N}, 8, 32>" if c2e == 32 else f"k_col_screen<{N}>") +
+                            " (y pass: FFT, transfer function, IFFT, screen phase, FFT, transfer function, IFFT)"}
+    # the committed counter summaries were taken with the kernels named above
+    # (the default shapes); another pick of the autotuner has no summary
+    default_shapes = c2e == 32 and not (ce == 32) and cc == 8
     traffic = None
     tr = read_traffic()
-    if tr and dom in tr:
+    if tr and dom in tr and default_shapes:
         traffic = tr[dom]
     valu = None
     vi = read_valu()
-    if vi and dom in vi:
+    if vi and dom in vi and default_shapes:
         rate = vi[dom] / (avg[dom] * 1e-3)
         valu = {"wave_instructions_per_launch": vi[dom], "achieved_per_s": round(rate, -6),
                 "peak_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
@@ -419,6 +515,10 @@ def main():
             # before HBM: wave instructions per launch (SQ_INSTS_VALU) over
             # the live launch duration, against the issue peak
             "valu": valu,
+            # the limiter the counters show: VALU issue when its fraction of
+            # the issue peak exceeds the kernel's fraction of HBM peak
+            "measured_limiter": (None if valu is None else
+                                 ("valu_issue" if valu["frac"] > achieved / HBM_PEAK_GBS else "hbm")),
             "avg_launch_ms": round(avg[dom], 5),
         },
         "fft_hbm_gbs": round(fft_gbs, 1),
@@ -434,15 +534,16 @@ def main():
                    "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),
                    "gap_passes": ctx.get_tuning("gap_passes"), "col2_e": ctx.get_tuning("col2_e"),
                    "graph": ctx.get_tuning("graph")},
+        "autotune": tune,
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
     }
     if not args.no_cpu:
         masters = [sim.master(l) for l in range(wl["L"])]
         sim.close()
-        cpu, errs = cpu_baseline_and_error(ctx, wl, masters, frames=args.cpu_frames,
-                                           threads=args.cpu_threads)
-        line["cpu_baseline"] = cpu
-        line["err_vs_fp64"] = errs
+        ms = _oracle_case(wl, masters, wl["N"])
+        del masters
+        line["err_vs_fp64"] = fp64_error(ctx, wl, ms, frames=args.cpu_frames, threads=args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(wl, ms, frames=args.cpu_frames, threads=args.cpu_threads)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

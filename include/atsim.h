@@ -64,7 +64,11 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             8192 / n, at most 8: two workgroups per CU)
  *   "col_c"   columns per column-pass workgroup (2, 4, 8, 16)
  *   "col_e"   elements per thread of the column Fresnel pass (16, or 32:
- *             8 columns per workgroup exchanging through half the LDS)
+ *             8 columns per workgroup exchanging through half the LDS).
+ *             32 applies only to Fresnel column passes with 256 <= n <= 2048
+ *             and col_c 0, 8 or 16; plain FFT passes, other sizes and
+ *             col_c 2 or 4 run the 16-element kernel (the value is kept and
+ *             atsim_get_tuning returns it)
  *   "streams" S = 2..4 (default 2): the Fresnel passes of S contiguous sets
  *             of the wavelengths run as S chains on S streams (set k started
  *             after set k - 1's first row pass) and join before the
@@ -238,6 +242,17 @@ int atsim_sim_profile_read(atsim_sim *sim, int kind, double *ms, long *count);
  * the context stream.  A diagnostic for tests, with no entry point in the
  * reference. */
 int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
+
+/* Autotuner of the launch shapes for this simulation's sizes: times `frames`
+ * frames (after two untimed ones) under each candidate of a coordinate
+ * descent over the tuning keys gap_passes, streams, row_mw, col2_e, col_c
+ * and col_e (two sweeps, values the kernels support at this grid size) and
+ * leaves the fastest set on the context.  Frames are written to d_pha /
+ * d_amp (as atsim_sim_frame); the layer positions are restored afterwards,
+ * so the series continues as if the search had not run.  report (nullable)
+ * receives one line per candidate, "key=value ... ms=<per frame>", and a
+ * final "best ..." line.  No counterpart in the reference. */
+int atsim_sim_autotune(atsim_sim *sim, float *d_pha, float *d_amp, int frames, char *report, long report_len);
 
 /* Output statistics of every frame (the OPD / scintillation statistics
  * stream; no counterpart in the reference, which leaves them to
