@@ -119,6 +119,28 @@ class Context:
         self.sync()
         return t
 
+    def fft2d_nm(self, t, sign=-1):
+        """in place, unnormalised, any (..., n1, n0) complex64 shape"""
+        assert t.is_contiguous()
+        n1, n0 = t.shape[-2], t.shape[-1]
+        nb = t.numel() // (n0 * n1)
+        self._prep(t)
+        self._chk(self.L.atsim_fft2d_nm(self.h, _ptr(t), n0, n1, nb, sign))
+        self.sync()
+        return t
+
+    def fresnel_nm(self, t_in, dx, z, lam, out=None):
+        """Fresnel_propagate_wavefront on any (..., n1, n0) complex64 shape"""
+        torch = _torch()
+        assert t_in.is_contiguous() and t_in.dtype == torch.complex64
+        n1, n0 = t_in.shape[-2], t_in.shape[-1]
+        nb = t_in.numel() // (n0 * n1)
+        out = torch.empty_like(t_in) if out is None else out
+        self._prep(t_in)
+        self._chk(self.L.atsim_fresnel_nm(self.h, _ptr(t_in), _ptr(out), n0, n1, nb, dx, z, lam))
+        self.sync()
+        return out
+
     def permut(self, t):
         assert t.is_contiguous()
         n1, n0 = t.shape[-2], t.shape[-1]

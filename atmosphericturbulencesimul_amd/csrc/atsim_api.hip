@@ -101,6 +101,54 @@ static void fresnel_table(float2* h, long n, double dx, double z, double lambda)
     }
 }
 
+// ------------------------------------------------------------------ transforms of any size
+// In-place radix-2 FFT of a power-of-two length in FP64 on the host (the
+// Bluestein filter spectrum), sign as the device transforms.
+static void host_fft(std::vector<double>& re, std::vector<double>& im, int sign) {
+    const size_t n = re.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < len / 2; k++) {
+                const double a = sign * 2.0 * M_PI * (double)k / (double)len;
+                const double wr = cos(a), wi = sin(a);
+                const size_t p = i + k, q = i + k + len / 2;
+                const double xr = re[q] * wr - im[q] * wi, xi = re[q] * wi + im[q] * wr;
+                re[q] = re[p] - xr;
+                im[q] = im[p] - xi;
+                re[p] += xr;
+                im[p] += xi;
+            }
+    }
+}
+
+// gather map of the reference's permut (fft.c:570-792) on an n0 x n1 image:
+// after the swap loops, element p holds what was at sigma[p]
+static std::vector<int> permut_sigma(long n0, long n1) {
+    std::vector<int> idx((size_t)(n0 * n1));
+    for (long k = 0; k < n0 * n1; k++) idx[(size_t)k] = (int)k;
+    const long xh = n0 / 2, yh = n1 / 2;
+    for (long jj = 0; jj < yh; jj++)
+        for (long ii = 0; ii < xh; ii++) std::swap(idx[(size_t)(jj * n0 + ii)], idx[(size_t)((jj + yh) * n0 + ii + xh)]);
+    for (long jj = yh; jj < n1; jj++)
+        for (long ii = 0; ii < xh; ii++) std::swap(idx[(size_t)(jj * n0 + ii)], idx[(size_t)((jj - yh) * n0 + ii + xh)]);
+    return idx;
+}
+
+static long next_pow2(long n) {
+    long m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
 // ------------------------------------------------------------------ allocation-free Fresnel
 // Device memory of atsim_fresnel_noalloc: the 1D transfer factor h(k) and the
 // twiddles e^{-2 pi i m / n} of one size, n <= ATSIM_NOALLOC_MAX_N (128 KB per
@@ -320,10 +368,113 @@ int atsim_fft2d(atsim_ctx* c, void* d_data, long n, long nbatch, int sign) {
 
 int atsim_permut(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch) {
     if (!c || !d_data) return ATSIM_EINVAL;
-    if (n0 <= 0 || n1 <= 0 || (n0 & 1) || (n1 & 1) || nbatch <= 0)
-        return fail(c, ATSIM_EUNSUPPORTED, "permut: device path needs even sizes (got %ld x %ld)", n0, n1);
-    HIPCHK(c, launch_permut((float2*)d_data, n0, n1, nbatch, c->stream));
-    return ATSIM_OK;
+    if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "permut: sizes must be > 0");
+    if (!(n0 & 1) && !(n1 & 1)) {
+        HIPCHK(c, launch_permut((float2*)d_data, n0, n1, nbatch, c->stream));
+        return ATSIM_OK;
+    }
+    // odd sizes: the swap loops are no longer a half-size shift; their
+    // gather map is built on the host and applied on the device
+    if (n0 * n1 > (1L << 30)) return fail(c, ATSIM_EUNSUPPORTED, "permut: image too large");
+    const std::vector<int> sig = permut_sigma(n0, n1);
+    int* d_sig = nullptr;
+    float2* tmp = nullptr;
+    const size_t bytes = sizeof(float2) * n0 * n1 * nbatch;
+    HIPCHK(c, hipMallocAsync((void**)&d_sig, sizeof(int) * sig.size(), c->stream));
+    int r = ATSIM_OK;
+    if (hipMallocAsync((void**)&tmp, bytes, c->stream) != hipSuccess) r = fail(c, ATSIM_ENOMEM, "permut: no memory");
+    if (r == ATSIM_OK && (hipMemcpyAsync(d_sig, sig.data(), sizeof(int) * sig.size(), hipMemcpyHostToDevice, c->stream) !=
+                              hipSuccess ||
+                          hipMemcpyAsync(tmp, d_data, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                          launch_index_gather(tmp, (float2*)d_data, d_sig, n0 * n1, nbatch, c->stream) != hipSuccess))
+        r = fail(c, ATSIM_EHIP, "permut: device gather failed");
+    if (tmp) hipFreeAsync(tmp, c->stream);
+    hipFreeAsync(d_sig, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the host map must outlive its copy
+    return r;
+}
+
+// One 1D transform of length L (unnormalised, sign) over nlines lines of a
+// batch of images (line base (l / per_img) img_size + (l % per_img) lstep,
+// elements estride apart), in place.  Power-of-two L: gather into padded
+// rows, row FFT, scatter.  Other L: Bluestein -- with c_n = e^{s i pi n^2 / L}
+// and d_m = e^{-s i pi m^2 / L}, X_k = c_k sum_n (x_n c_n) d_{k-n}, the
+// convolution taken by FFTs of length M >= 2L - 1 (filter spectrum built on
+// the host in FP64, 1/M folded in).  Chirp angles use n^2 mod 2L (exact).
+static int axis_transform(atsim_ctx* c, float2* data, long L, long nlines, long per_img, long img_size, long lstep,
+                          long estride, int sign) {
+    if (L == 1) return ATSIM_OK;
+    const bool p2 = pow2(L);
+    const long M = p2 ? L : next_pow2(2 * L - 1);
+    // the device row transforms start at 16: shorter lines take the padded
+    // (Bluestein) path at length 16 >= 2 L - 1
+    const long Mf = std::max<long>(M, 16);
+    if (Mf > 8192)
+        return fail(c, ATSIM_EUNSUPPORTED, "fft: axis length %ld needs a transform of %ld > 8192", L, Mf);
+    const bool blue = !p2 || M < 16;
+    const long F = (nlines + Mf - 1) / Mf;
+    if (F > 65535) return fail(c, ATSIM_EUNSUPPORTED, "fft: %ld lines of length %ld are too many", nlines, L);
+    float2* tw = nullptr;
+    if (int r = get_twiddles(c, Mf, &tw)) return r;
+    float2 *scr = nullptr, *d_c = nullptr, *d_h = nullptr;
+    HIPCHK(c, hipMallocAsync((void**)&scr, sizeof(float2) * F * Mf * Mf, c->stream));
+    std::vector<float2> hc, hh;
+    int r = ATSIM_OK;
+    if (blue) {
+        hc.resize((size_t)L);
+        std::vector<double> dr((size_t)Mf, 0.0), di((size_t)Mf, 0.0);
+        for (long n = 0; n < L; n++) {
+            const long q = (long)(((unsigned long long)n * (unsigned long long)n) % (unsigned long long)(2 * L));
+            const double a = M_PI * (double)q / (double)L;
+            hc[(size_t)n] = make_float2((float)cos(a), (float)(sign * sin(a)));
+            dr[(size_t)n] = cos(a);
+            di[(size_t)n] = -sign * sin(a);
+            if (n > 0) {
+                dr[(size_t)(Mf - n)] = cos(a);
+                di[(size_t)(Mf - n)] = -sign * sin(a);
+            }
+        }
+        host_fft(dr, di, -1);
+        hh.resize((size_t)Mf);
+        for (long k = 0; k < Mf; k++) hh[(size_t)k] = make_float2((float)(dr[(size_t)k] / Mf), (float)(di[(size_t)k] / Mf));
+        if (hipMallocAsync((void**)&d_c, sizeof(float2) * L, c->stream) != hipSuccess ||
+            hipMallocAsync((void**)&d_h, sizeof(float2) * Mf, c->stream) != hipSuccess ||
+            hipMemcpyAsync(d_c, hc.data(), sizeof(float2) * L, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(d_h, hh.data(), sizeof(float2) * Mf, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            r = fail(c, ATSIM_EHIP, "fft: table upload failed");
+    }
+    ScreenArgs none{};
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    if (r == ATSIM_OK) {
+        bool good = ok(launch_line_gather(data, scr, nlines, L, Mf, per_img, img_size, lstep, estride, d_c, c->stream));
+        if (blue) {
+            good = good && ok(launch_row(Mf, OP_FFT, -1, false, scr, F, nullptr, tw, none, c->stream)) &&
+                   ok(launch_row_mul(scr, d_h, F * Mf, Mf, c->stream)) &&
+                   ok(launch_row(Mf, OP_FFT, +1, false, scr, F, nullptr, tw, none, c->stream));
+        } else {
+            good = good && ok(launch_row(Mf, OP_FFT, sign, false, scr, F, nullptr, tw, none, c->stream));
+        }
+        good = good && ok(launch_line_scatter(scr, data, nlines, L, Mf, per_img, img_size, lstep, estride, d_c,
+                                              c->stream));
+        if (!good) r = fail(c, ATSIM_EHIP, "fft: launch failed for length %ld", L);
+    }
+    hipFreeAsync(scr, c->stream);
+    if (d_c) hipFreeAsync(d_c, c->stream);
+    if (d_h) hipFreeAsync(d_h, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host tables must outlive their copies
+    return r;
+}
+
+int atsim_fft2d_nm(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch, int sign) {
+    if (!c || !d_data) return ATSIM_EINVAL;
+    if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "fft2d_nm: sizes must be > 0");
+    if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "fft2d_nm: sign must be -1 or +1");
+    if (n0 == n1 && supported_n(n0) && nbatch <= 65535) return atsim_fft2d(c, d_data, n0, nbatch, sign);
+    float2* d = (float2*)d_data;
+    // along x: lines are the rows, n1 per image
+    if (int r = axis_transform(c, d, n0, n1 * nbatch, n1, n0 * n1, n0, 1, sign)) return r;
+    // along y: lines are the columns, n0 per image
+    return axis_transform(c, d, n1, n0 * nbatch, n0, n0 * n1, 1, n0, sign);
 }
 
 // ------------------------------------------------------------------ Fresnel
@@ -353,6 +504,38 @@ int atsim_fresnel(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbat
     int r = fresnel_run(c, (float2*)d_out, n, nbatch, d_tab);
     hipFreeAsync(d_tab, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host table must outlive the copy
+    return r;
+}
+
+int atsim_fresnel_nm(atsim_ctx* c, const void* d_in, void* d_out, long n0, long n1, long nbatch, double dx,
+                     double z, double lambda) {
+    if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "fresnel_nm: sizes must be > 0");
+    if (!(dx > 0.0)) return fail(c, ATSIM_EINVAL, "fresnel_nm: pupil scale must be > 0");
+    if (n0 == n1 && supported_n(n0) && nbatch <= 65535) return atsim_fresnel(c, d_in, d_out, n0, nbatch, dx, z, lambda);
+    const size_t bytes = sizeof(float2) * n0 * n1 * nbatch;
+    float2* tmp = nullptr;
+    int* d_sig = nullptr;
+    std::vector<int> sig;
+    HIPCHK(c, hipMallocAsync((void**)&tmp, bytes, c->stream));
+    int r = ATSIM_OK;
+    if (hipMemcpyAsync(tmp, d_in, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "fresnel_nm: copy failed");
+    if (r == ATSIM_OK) r = atsim_fft2d_nm(c, tmp, n0, n1, nbatch, -1);
+    if (r == ATSIM_OK && ((n0 & 1) || (n1 & 1))) {
+        sig = permut_sigma(n0, n1);
+        if (hipMallocAsync((void**)&d_sig, sizeof(int) * sig.size(), c->stream) != hipSuccess ||
+            hipMemcpyAsync(d_sig, sig.data(), sizeof(int) * sig.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            r = fail(c, ATSIM_EHIP, "fresnel_nm: permut map upload failed");
+    }
+    // WFpropagate.c:97: the coefficient uses naxes[0] for both axes
+    const double coeff = M_PI * z * lambda / (dx * n0) / (dx * n0);
+    if (r == ATSIM_OK && launch_fresnel_any(tmp, (float2*)d_out, n0, n1, nbatch, coeff, d_sig, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "fresnel_nm: launch failed");
+    if (r == ATSIM_OK) r = atsim_fft2d_nm(c, d_out, n0, n1, nbatch, +1);
+    hipFreeAsync(tmp, c->stream);
+    if (d_sig) hipFreeAsync(d_sig, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return r;
 }
 

@@ -899,6 +899,122 @@ hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ transforms of any size
+// A 1D transform of length L along one axis of a batch of images is run on
+// contiguous rows of a padded scratch [rows][M] (M a power of two): k_line_gather
+// copies line l, element n (n < L) from src + base(l) + n estride, times
+// chirp[n] when given, and zeroes n in [L, M); k_line_scatter writes element
+// k < L back, times chirp[k] when given.  base(l) = (l / per_img) img_size +
+// (l % per_img) lstep.  Lines are the fast index of the launch so that lines
+// adjacent in memory (the columns of an image) load together.
+__global__ void k_line_gather(const float2* __restrict__ src, float2* __restrict__ dst, long nlines, long L, long M,
+                              long per_img, long img_size, long lstep, long estride,
+                              const float2* __restrict__ chirp) {
+    const long total = nlines * M;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long n = e / nlines, l = e - n * nlines;
+        float2 v = make_float2(0.f, 0.f);
+        if (n < L) {
+            const long base = (l / per_img) * img_size + (l % per_img) * lstep;
+            v = src[base + n * estride];
+            if (chirp) v = cmul(v, chirp[n]);
+        }
+        dst[l * M + n] = v;
+    }
+}
+
+__global__ void k_line_scatter(const float2* __restrict__ src, float2* __restrict__ dst, long nlines, long L, long M,
+                               long per_img, long img_size, long lstep, long estride,
+                               const float2* __restrict__ chirp) {
+    const long total = nlines * L;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long k = e / nlines, l = e - k * nlines;
+        float2 v = src[l * M + k];
+        if (chirp) v = cmul(v, chirp[k]);
+        dst[(l / per_img) * img_size + (l % per_img) * lstep + k * estride] = v;
+    }
+}
+
+// data[r][k] *= h[k] for rows r < nrows of length M
+__global__ void k_row_mul(float2* __restrict__ data, const float2* __restrict__ h, long nrows, long M) {
+    const long total = nrows * M;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+        data[e] = cmul(data[e], h[e % M]);
+}
+
+// out[b][p] = in[b][sigma[p]] over nbatch images of n elements
+__global__ void k_index_gather(const float2* __restrict__ in, float2* __restrict__ out, const int* __restrict__ sigma,
+                               long n, long nbatch) {
+    const long total = n * nbatch;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long b = e / n, p = e - b * n;
+        out[e] = in[b * n + sigma[p]];
+    }
+}
+
+// The Fresnel factor of Fresnel_propagate_wavefront (WFpropagate.c:68-152) on
+// the spectrum of any n0 x n1 image: permut, times e^{-i coeff ((ii - n0/2)^2 +
+// (jj - n1/2)^2)} / (n0 n1) at the permuted position (ii, jj), permut.  With
+// sigma the gather map of one permut (out[p] = in[sigma(p)]): out[p] =
+// in[sigma(sigma(p))] h(sigma(p)).  sigma null: even sizes, where permut is
+// the half-size shift and sigma(sigma(p)) = p.  Angle and factor in FP64.
+__global__ void k_fresnel_any(const float2* __restrict__ in, float2* __restrict__ out, long n0, long n1, long nbatch,
+                              double coeff, const int* __restrict__ sigma) {
+    const long n = n0 * n1, total = n * nbatch;
+    const double inv = 1.0 / ((double)n0 * (double)n1);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long b = e / n, p = e - b * n;
+        long q, src;
+        if (sigma) {
+            q = sigma[p];
+            src = sigma[q];
+        } else {
+            const long jj = p / n0, ii = p - jj * n0;
+            q = ((jj + n1 / 2) % n1) * n0 + (ii + n0 / 2) % n0;
+            src = p;
+        }
+        const long jq = q / n0, iq = q - jq * n0;
+        const long di = iq - n0 / 2, dj = jq - n1 / 2;
+        const double ang = -coeff * (double)(di * di + dj * dj);
+        double sn, cs;
+        sincos(ang, &sn, &cs);
+        const float2 v = in[b * n + src];
+        const double re = (double)v.x * inv, im = (double)v.y * inv;
+        out[e] = make_float2((float)(re * cs - im * sn), (float)(re * sn + im * cs));
+    }
+}
+
+static unsigned grid_for(long total) {
+    long g = (total + 255) / 256;
+    return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+hipError_t launch_line_gather(const float2* src, float2* dst, long nlines, long L, long M, long per_img, long img_size,
+                              long lstep, long estride, const float2* chirp, hipStream_t st) {
+    k_line_gather<<<grid_for(nlines * M), 256, 0, st>>>(src, dst, nlines, L, M, per_img, img_size, lstep, estride,
+                                                         chirp);
+    return hipGetLastError();
+}
+hipError_t launch_line_scatter(const float2* src, float2* dst, long nlines, long L, long M, long per_img,
+                               long img_size, long lstep, long estride, const float2* chirp, hipStream_t st) {
+    k_line_scatter<<<grid_for(nlines * L), 256, 0, st>>>(src, dst, nlines, L, M, per_img, img_size, lstep, estride,
+                                                          chirp);
+    return hipGetLastError();
+}
+hipError_t launch_row_mul(float2* data, const float2* h, long nrows, long M, hipStream_t st) {
+    k_row_mul<<<grid_for(nrows * M), 256, 0, st>>>(data, h, nrows, M);
+    return hipGetLastError();
+}
+hipError_t launch_index_gather(const float2* in, float2* out, const int* sigma, long n, long nbatch, hipStream_t st) {
+    k_index_gather<<<grid_for(n * nbatch), 256, 0, st>>>(in, out, sigma, n, nbatch);
+    return hipGetLastError();
+}
+hipError_t launch_fresnel_any(const float2* in, float2* out, long n0, long n1, long nbatch, double coeff,
+                              const int* sigma, hipStream_t st) {
+    k_fresnel_any<<<grid_for(n0 * n1 * nbatch), 256, 0, st>>>(in, out, n0, n1, nbatch, coeff, sigma);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ noise
 __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll

@@ -46,6 +46,20 @@ hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float
                       const float2* tw, hipStream_t st, int c = 0, int e = 0, int wlo = 0, int wcnt = 0);
 
 hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st);
+// transforms of any size: lines of length L (< = M) between strided images and
+// a padded scratch [nlines][M]; base of line l = (l / per_img) img_size +
+// (l % per_img) lstep, elements estride apart; chirp (nullable) multiplies
+hipError_t launch_line_gather(const float2* src, float2* dst, long nlines, long L, long M, long per_img, long img_size,
+                              long lstep, long estride, const float2* chirp, hipStream_t st);
+hipError_t launch_line_scatter(const float2* src, float2* dst, long nlines, long L, long M, long per_img,
+                               long img_size, long lstep, long estride, const float2* chirp, hipStream_t st);
+hipError_t launch_row_mul(float2* data, const float2* h, long nrows, long M, hipStream_t st);
+// out[b][p] = in[b][sigma[p]] (images of n elements)
+hipError_t launch_index_gather(const float2* in, float2* out, const int* sigma, long n, long nbatch, hipStream_t st);
+// Fresnel factor of any n0 x n1 spectrum with the reference's permut pair
+// (sigma: gather map of one permut, null for even sizes); out != in
+hipError_t launch_fresnel_any(const float2* in, float2* out, long n0, long n1, long nbatch, double coeff,
+                              const int* sigma, hipStream_t st);
 hipError_t launch_noise(float2* out, long n, uint64_t seed, uint32_t stream, hipStream_t st);
 hipError_t launch_synth_filter(const float2* noise, float2* out, long size, double f0, double fi,
                                double rlim, hipStream_t st);

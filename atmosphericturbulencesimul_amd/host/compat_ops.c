@@ -86,11 +86,11 @@ static long fft_common(const char *in_name, const char *out_name, int sign) {
     if (!in) return atsim_compat_fail(0, "do2dfft: image \"%s\" does not exist", in_name);
     if (!is_complex(in->md->atype)) return atsim_compat_fail(0, "do2dfft: image \"%s\" is not complex", in_name);
     long n0, n1, nb;
-    if (check_square_pow2("do2dfft", in->md->size[0], in->md->naxis > 1 ? in->md->size[1] : 1)) return -1;
     void *d = to_device_c64(in, &n0, &n1, &nb);
     if (!d) return -1;
     atsim_ctx *c = atsim_compat_ctx();
-    int rc = atsim_fft2d(c, d, n0, nb, sign);
+    /* any shape, as FFTW plans it (fft.c:1190) */
+    int rc = atsim_fft2d_nm(c, d, n0, n1, nb, sign);
     long ID = -1;
     if (rc == 0) {
         ID = new_like(out_name, in, in->md->atype);
@@ -112,7 +112,6 @@ int permut(const char *ID_name) {
     if (!c) return -1;
     atsim_image_metadata *md = im->md;
     const long n0 = md->size[0], n1 = md->naxis > 1 ? md->size[1] : 1, nb = md->naxis > 2 ? md->size[2] : 1;
-    if ((n0 & 1) || (n1 & 1)) return atsim_compat_fail(0, "permut: the device path needs even sizes");
     const long n = md->nelement;
     /* real images travel as (value, 0) pairs */
     float *h = (float *)malloc((size_t)n * 8);
@@ -185,12 +184,11 @@ int Fresnel_propagate_wavefront(const char *in, const char *out, double PUPIL_SC
     atsim_image *im = atsim_reg_get(image_ID(in));
     if (!im) return atsim_compat_fail(0, "Fresnel_propagate_wavefront: image \"%s\" does not exist", in);
     if (!is_complex(im->md->atype)) return atsim_compat_fail(0, "Fresnel_propagate_wavefront: \"%s\" is not complex", in);
-    if (check_square_pow2("Fresnel_propagate_wavefront", im->md->size[0], im->md->size[1])) return -1;
     long n0, n1, nb;
     void *d = to_device_c64(im, &n0, &n1, &nb);
     if (!d) return -1;
     atsim_ctx *c = atsim_compat_ctx();
-    int rc = atsim_fresnel(c, d, d, n0, nb, PUPIL_SCALE, z, lambda);
+    int rc = atsim_fresnel_nm(c, d, d, n0, n1, nb, PUPIL_SCALE, z, lambda);
     if (rc == 0) {
         long ID = new_like(out, im, im->md->atype);
         atsim_image *o = atsim_reg_get(ID);

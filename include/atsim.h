@@ -101,8 +101,17 @@ int atsim_device_ok(atsim_ctx *ctx);
  * sign = -1 forward, +1 backward: the reference's do2dfft / do2dffti
  * (src/fft/fft.c:1298-1315, FFTW plan over the swapped axes fft.c:1178-1190). */
 int atsim_fft2d(atsim_ctx *ctx, void *d_data, long n, long nbatch, int sign);
-/* Quadrant swap of nbatch complex n0 x n1 images, permut (fft.c:570-792);
- * even sizes only on the device. */
+/* 2D c2c of any nbatch contiguous n0 x n1 images (n0 = x size, fastest):
+ * the reference's FFT_do2dfft on any shape (FFTW plan over the swapped axes,
+ * fft.c:1145-1209), unnormalised, sign as atsim_fft2d.  Square power-of-two
+ * images take atsim_fft2d; otherwise each axis is transformed on its own,
+ * power-of-two lengths directly and other lengths by Bluestein's chirp
+ * convolution through power-of-two transforms (lengths up to 4096, or powers
+ * of two up to 8192).  Allocates temporary device memory. */
+int atsim_fft2d_nm(atsim_ctx *ctx, void *d_data, long n0, long n1, long nbatch, int sign);
+/* Quadrant swap of nbatch complex n0 x n1 images, permut (fft.c:570-792),
+ * any size: for odd sizes the reference's swap loops are applied as their
+ * gather map. */
 int atsim_permut(atsim_ctx *ctx, void *d_data, long n0, long n1, long nbatch);
 
 /* ---------------------------------------------------------------- Fresnel */
@@ -121,6 +130,12 @@ int atsim_fresnel(atsim_ctx *ctx, const void *d_in, void *d_out, long n, long nb
  * context stream (atsim_sync waits); same results as atsim_fresnel. */
 int atsim_fresnel_noalloc(atsim_ctx *ctx, const void *d_in, void *d_out, long n, long nbatch,
                           double pupil_scale, double z, double lambda);
+/* Fresnel_propagate_wavefront on any n0 x n1 shape (WFpropagate.c:68-152):
+ * FFT, permut, e^{-i coeff ((ii - n0/2)^2 + (jj - n1/2)^2)} / (n0 n1) with
+ * coeff = pi z lambda / (dx n0)^2 (n0 for both axes, as the reference),
+ * permut, inverse FFT.  Square power-of-two images take atsim_fresnel. */
+int atsim_fresnel_nm(atsim_ctx *ctx, const void *d_in, void *d_out, long n0, long n1, long nbatch,
+                     double pupil_scale, double z, double lambda);
 /* Init_Fresnel_propagate_wavefront (WFpropagate.c:166-261): fills the
  * anti-aliased, permuted transfer function C (n x n complex, device). */
 int atsim_fresnel_tf_init(atsim_ctx *ctx, void *d_C, long n, double pupil_scale, double z,

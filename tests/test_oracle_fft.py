@@ -92,3 +92,14 @@ def test_permut_even_is_fftshift_involution(oracle, shape):
 def test_permut_odd_follows_swap_loops(oracle, shape):
     x = _rand(shape, 7)
     assert np.array_equal(oracle.permut(x), _permut_loops(x))
+
+
+@pytest.mark.parametrize("shape", [(12, 20), (9, 15), (64, 48), (1, 30), (17, 1)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft2d_rectangular_and_odd_matches_numpy(oracle, shape, sign):
+    # FFTW's 2D plan over any (n1, n0) shape (fft.c:1190): numpy's pocketfft
+    # as the independent FP64 check
+    rng = np.random.default_rng(sum(shape) + sign)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    want = np.fft.fft2(x) if sign < 0 else np.fft.ifft2(x) * x.size
+    assert np.allclose(oracle.fft2d(x, sign), want, atol=1e-9)

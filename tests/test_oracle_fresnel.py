@@ -165,3 +165,34 @@ def test_init_tf_mask_rim_over_many_radii(oracle):
         C = oracle.fresnel_init_tf(size, 0.01, 0.0, 1e-6, rad)
         k = sum((x < rad).astype(int) for x in d)
         assert np.array_equal(C, np.where(k >= 5, 1.0 / size ** 2, 0.0).astype(complex)), rad
+
+
+@pytest.mark.parametrize("shape", [(32, 48), (20, 20), (15, 21), (24, 9)])
+def test_rectangular_and_odd_fresnel_follows_the_reference_steps(oracle, shape):
+    # Fresnel_propagate_wavefront on any shape (WFpropagate.c:68-152): forward
+    # FFT, permut, e^{-i coeff ((ii - n0/2)^2 + (jj - n1/2)^2)} / (n0 n1) with
+    # coeff from naxes[0] on both axes, permut, inverse FFT -- built here from
+    # numpy's FFT and the swap loops of permut applied to an index array
+    n1, n0 = shape
+    dx, z, lam = 0.02, 3000.0, 1e-6
+    rng = np.random.default_rng(n0 * n1)
+    E = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    def permut_np(a):
+        idx = np.arange(a.size).reshape(a.shape)
+        xh, yh = n0 // 2, n1 // 2
+        for jj in range(yh):
+            for ii in range(xh):
+                idx[jj, ii], idx[jj + yh, ii + xh] = idx[jj + yh, ii + xh], idx[jj, ii]
+        for jj in range(yh, n1):
+            for ii in range(xh):
+                idx[jj, ii], idx[jj - yh, ii + xh] = idx[jj - yh, ii + xh], idx[jj, ii]
+        return a.ravel()[idx.ravel()].reshape(a.shape)
+
+    t = permut_np(np.fft.fft2(E))
+    coeff = np.pi * z * lam / (dx * n0) ** 2
+    jj, ii = np.mgrid[0:n1, 0:n0]
+    t = t * np.exp(-1j * coeff * ((ii - n0 // 2) ** 2 + (jj - n1 // 2) ** 2)) / (n0 * n1)
+    want = np.fft.ifft2(permut_np(t)) * (n0 * n1)
+    assert np.allclose(oracle.fresnel(E, dx, z, lam), want, atol=1e-10)
+    assert np.allclose(oracle.permut(E), permut_np(E))
