@@ -80,6 +80,8 @@ _SIGS = {
                                               C.POINTER(C.c_double)]),
     "atsim_sim_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
     "atsim_sim_layer_screens": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atsim_sims_frame": (C.c_int, [C.POINTER(C.c_void_p), C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "atsim_sim_use_masters": (C.c_int, [C.c_void_p, C.c_void_p]),
     "atsim_sim_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_long]),
     "atsim_sim_stats_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "atsim_sim_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_long]),

@@ -384,6 +384,11 @@ class Simulation:
         self.tune_report = lines
         return dict(best, ms_per_frame=ms, candidates=len(lines) - 1, cached=False)
 
+    def use_masters(self, owner):
+        """share the masters of another simulation of this context (same M and L)"""
+        self.ctx._chk(self.ctx.L.atsim_sim_use_masters(self.h, owner.h))
+        self._masters_owner = owner  # keep it alive
+
     def stats_enable(self, enable=True):
         """reduce every following frame's outputs to per-field statistics"""
         self.ctx._chk(self.ctx.L.atsim_sim_stats_enable(self.h, int(enable)))
@@ -430,3 +435,17 @@ class Simulation:
         n = C.c_long()
         self.ctx._chk(self.ctx.L.atsim_sim_profile_read(self.h, self.PROF_KINDS[kind], C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def sims_frame(sims, sync=True):
+    """one frame of several simulations of one context (any grid sizes) in one
+    call (atsim_sims_frame): each simulation's own (pha, amp) buffers"""
+    ctx = sims[0].ctx
+    n = len(sims)
+    hs = (C.c_void_p * n)(*[s.h.value if isinstance(s.h, C.c_void_p) else s.h for s in sims])
+    ph = (C.c_void_p * n)(*[s.pha.data_ptr() for s in sims])
+    am = (C.c_void_p * n)(*[s.amp.data_ptr() for s in sims])
+    ctx._chk(ctx.L.atsim_sims_frame(hs, n, ph, am))
+    if sync:
+        ctx.sync()
+    return [(s.pha, s.amp) for s in sims]

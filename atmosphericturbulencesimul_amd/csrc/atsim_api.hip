@@ -39,6 +39,9 @@ struct atsim_ctx {
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ready[ATSIM_COPY_SLOTS] = {};
     hipEvent_t copy_done[ATSIM_COPY_SLOTS] = {};
+    // frames of several simulations in one call (atsim_sims_frame)
+    long mg_frames = 0;
+    hipEvent_t mg_fork = nullptr;
     double* nn_part = nullptr;  // 256 partial sums of the network's loss reduction
     float* nn_bias = nullptr;   // per-chunk partial sums of bias gradients
     size_t nn_bias_cap = 0;
@@ -202,6 +205,7 @@ void atsim_ctx_destroy(atsim_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (auto& kv : c->twiddles) hipFree(kv.second);
+    if (c->mg_fork) hipEventDestroy(c->mg_fork);
     if (c->nn_part) hipFree(c->nn_part);
     if (c->nn_bias) hipFree(c->nn_bias);
     if (c->copy_stream) {
@@ -793,6 +797,10 @@ struct atsim_sim {
     bool stats = false;
     double* st_part = nullptr;
     double* st_out = nullptr;
+    // own stream and join event of this simulation inside a multi-simulation frame
+    hipStream_t own = nullptr;
+    hipEvent_t own_done = nullptr;
+    bool masters_borrowed = false;  // masters belong to another simulation (atsim_sim_use_masters)
     // per-launch event timing (bench instrumentation)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -864,7 +872,9 @@ static void sim_free(atsim_sim* s) {
     if (!s) return;
     if (s->ctx) hipStreamSynchronize(s->ctx->stream);
     sim_drop_graph(s);
-    hipFree(s->masters);
+    if (s->own) hipStreamDestroy(s->own);
+    if (s->own_done) hipEventDestroy(s->own_done);
+    if (!s->masters_borrowed) hipFree(s->masters);
     hipFree(s->fields);
     hipFree(s->tf);
     hipFree(s->ls_dev);
@@ -1176,9 +1186,11 @@ static hipError_t launch_step(atsim_sim* s, bool fused, long step, ScreenArgs sa
                              c->col2_e);
 }
 
-// the kernels of one frame (positions already uploaded), in stream order
-static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
+// the kernels of one frame (positions already uploaded), in stream order on
+// ms (the context stream unless a multi-simulation frame gives another)
+static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field, hipStream_t ms = nullptr) {
     atsim_ctx* c = s->ctx;
+    if (!ms) ms = c->stream;
     const long L = s->L, D = s->D, W = s->W, M = s->M;
     ScreenArgs sa{};
     sa.masters = s->masters;
@@ -1202,12 +1214,12 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         // runs beside another's column pass (memory-bound).  All join the
         // context stream before extraction.  The phase planes of the column
         // passes are sampled first, on the context stream.
-        if (fused) HIPCHK(c, phase(c->stream));
-        HIPCHK(c, hipEventRecord(s->ev_fork, c->stream));
+        if (fused) HIPCHK(c, phase(ms));
+        HIPCHK(c, hipEventRecord(s->ev_fork, ms));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
         for (long step = 0; step < nsteps; step++) {
             for (int k = 0; k < nsets; k++) {
-                hipStream_t st = k ? s->xs_stream[k - 1] : c->stream;
+                hipStream_t st = k ? s->xs_stream[k - 1] : ms;
                 sa.wlo = (int)(W * k / nsets);
                 sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
                 HIPCHK(c, launch_step(s, fused, step, sa, st));
@@ -1220,17 +1232,17 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         sa.wlo = sa.wcnt = 0;
         for (int k = 1; k < nsets; k++) {
             HIPCHK(c, hipEventRecord(s->ev_join[k - 1], s->xs_stream[k - 1]));
-            HIPCHK(c, hipStreamWaitEvent(c->stream, s->ev_join[k - 1], 0));
+            HIPCHK(c, hipStreamWaitEvent(ms, s->ev_join[k - 1], 0));
         }
     } else if (s->propagate) {
         if (fused) {
             long pp = prof_begin(s, PROF_PHASE);
-            HIPCHK(c, phase(c->stream));
+            HIPCHK(c, phase(ms));
             prof_end(s, pp);
         }
         for (long step = 0; step < nsteps; step++) {
             long pr = prof_begin(s, step_kind(fused, step, G));
-            HIPCHK(c, launch_step(s, fused, step, sa, c->stream));
+            HIPCHK(c, launch_step(s, fused, step, sa, ms));
             prof_end(s, pr);
         }
     }
@@ -1259,20 +1271,20 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     a.D = (int)D;
     a.amplitude = s->amplitude;
     a.xs = s->xs;
-    if (s->nblockfields > 0 && s->xs > 0) HIPCHK(c, launch_zero_f64(s->bsum, F * s->xs * s->xs * 2, c->stream));
+    if (s->nblockfields > 0 && s->xs > 0) HIPCHK(c, launch_zero_f64(s->bsum, F * s->xs * s->xs * 2, ms));
     long pe = prof_begin(s, PROF_EXTRACT);
-    HIPCHK(c, launch_extract(a, d_field != nullptr, c->stream));
+    HIPCHK(c, launch_extract(a, d_field != nullptr, ms));
     prof_end(s, pe);
     if (s->nblockfields > 0) {
         long pb = prof_begin(s, PROF_BLOCK);
-        HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->cmax, s->flist_dev, s->nblockfields, s->xs, c->stream));
+        HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->cmax, s->flist_dev, s->nblockfields, s->xs, ms));
         HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields, (int)s->Nout,
-                                     s->xs, c->stream));
+                                     s->xs, ms));
         prof_end(s, pb);
     }
     if (s->stats) {
         long ps = prof_begin(s, PROF_STATS);
-        HIPCHK(c, launch_stats(d_pha, d_amp, F, s->Nout * s->Nout, s->st_part, s->st_out, c->stream));
+        HIPCHK(c, launch_stats(d_pha, d_amp, F, s->Nout * s->Nout, s->st_part, s->st_out, ms));
         prof_end(s, ps);
     }
     return ATSIM_OK;
@@ -1326,6 +1338,55 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     gs->used = s->frames_done;
     HIPCHK(c, hipGraphLaunch(gs->exec, c->stream));
     s->frames_done++;
+    return ATSIM_OK;
+}
+
+int atsim_sims_frame(atsim_sim** sims, long n, float** d_pha, float** d_amp) {
+    if (!sims || n <= 0 || !d_pha || !d_amp) return ATSIM_EINVAL;
+    atsim_ctx* c = sims[0] ? sims[0]->ctx : nullptr;
+    if (!c) return ATSIM_EINVAL;
+    for (long k = 0; k < n; k++) {
+        if (!sims[k] || sims[k]->ctx != c || !d_pha[k] || !d_amp[k])
+            return fail(c, ATSIM_EINVAL, "sims_frame: simulation %ld is null, has no outputs or another context", k);
+        for (long q = 0; q < k; q++)
+            if (sims[q] == sims[k]) return fail(c, ATSIM_EINVAL, "sims_frame: simulation %ld appears twice", k);
+        if (sims[k]->prof) return fail(c, ATSIM_EINVAL, "sims_frame: per-launch profiling is on for simulation %ld", k);
+    }
+    for (long k = 0; k < n; k++) {
+        atsim_sim* s = sims[k];
+        if (!s->own) {
+            HIPCHK(c, hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking));
+            HIPCHK(c, hipEventCreateWithFlags(&s->own_done, hipEventDisableTiming));
+        }
+    }
+    if (!c->mg_fork) HIPCHK(c, hipEventCreateWithFlags(&c->mg_fork, hipEventDisableTiming));
+    for (long k = 0; k < n; k++)
+        if (int r = advance_positions(sims[k])) return r;
+    // every simulation's frame on its own stream, forked from and joined to
+    // the context stream: frames of different grid sizes run side by side.
+    // The launches are eager (this path captures no graph).
+    HIPCHK(c, hipEventRecord(c->mg_fork, c->stream));
+    for (long k = 0; k < n; k++) {
+        atsim_sim* s = sims[k];
+        HIPCHK(c, hipStreamWaitEvent(s->own, c->mg_fork, 0));
+        if (int r = enqueue_frame(s, d_pha[k], d_amp[k], nullptr, s->own)) return r;
+        HIPCHK(c, hipEventRecord(s->own_done, s->own));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, s->own_done, 0));
+    }
+    c->mg_frames++;
+    return ATSIM_OK;
+}
+
+int atsim_sim_use_masters(atsim_sim* s, const atsim_sim* owner) {
+    if (!s || !owner || s == owner) return ATSIM_EINVAL;
+    atsim_ctx* c = s->ctx;
+    if (owner->ctx != c || owner->M != s->M || owner->L != s->L)
+        return fail(c, ATSIM_EINVAL, "use_masters: the owner needs the same context, M and L");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!s->masters_borrowed) hipFree(s->masters);
+    s->masters = owner->masters;
+    s->masters_borrowed = true;
+    sim_drop_graph(s);
     return ATSIM_OK;
 }
 
@@ -1465,7 +1526,9 @@ int atsim_sim_autotune(atsim_sim* s, float* d_pha, float* d_amp, int frames, cha
                 double ms = 0.0;
                 r = measure(t, &ms);
                 if (r) break;
-                if (ms < best_ms) {
+                // a new shape has to beat the current one by more than the
+                // timing noise of a few frames (1 %)
+                if (ms < 0.99 * best_ms) {
                     best_ms = ms;
                     best = t;
                     changed = true;

@@ -246,6 +246,90 @@ def cpu_baseline(wl, ms, frames=1, threads=0):
     }
 
 
+def mixed_sizes(ctx, wl, owner, steps):
+    """The batched uneven-size workload: 512^2, 1024^2 and 2048^2 simulations
+    (16 directions on a 4 x 4 grid at -1, -1/3, 1/3, 1 arcmin, 8 wavelengths,
+    10 layers) over the headline's 8192^2 masters.  Rates in wavefront-frames/s
+    (384 per step): all three sizes in one call (atsim_sims_frame, each size
+    on its own stream, eager launches), the three simulations one after the
+    other, and the plain per-wavelength launch (one field per simulation;
+    timed on one direction per size and scaled to the 16)."""
+    import torch
+    from atmosphericturbulencesimul_amd.api import Simulation, superlayers, sims_frame
+    sizes = (512, 1024, 2048)
+    L, M = wl["L"], wl["M"]
+    sa, sli = superlayers(wl["alts"], wl["cn2"], wl["bin_m"])
+    ang = np.array([-1.0, -1.0 / 3, 1.0 / 3, 1.0]) * math.pi / (180 * 60)
+    dirs = [(tx, ty) for ty in ang for tx in ang]
+    D, W = len(dirs), wl["W"]
+    vx = wl["speeds"] * np.cos(wl["dirs"]) / wl["dx"]
+    vy = wl["speeds"] * np.sin(wl["dirs"]) / wl["dx"]
+    scoeff = np.array([[wl["lam_ref"] / lam] * L for lam in wl["lambdas"]])
+    unwrap = [0 if abs(lam - wl["lam_ref"]) < 1e-12 else 1 for lam in wl["lambdas"]]
+
+    def make(N, dset, lams, sc, uw):
+        x0 = np.array([[0.05 * M + tx * a / wl["dx"] for a in wl["alts"]] for tx, _ in dset])
+        y0 = np.array([[0.05 * M + ty * a / wl["dx"] for a in wl["alts"]] for _, ty in dset])
+        sim = Simulation(ctx, N, N, M, sa, sli, lams, sc, uw, x0, y0, vx, vy, wl["dx"], wl["dt"], 0.0, True)
+        sim.use_masters(owner)
+        return sim
+
+    sims = [make(N, dirs, wl["lambdas"], scoeff, unwrap) for N in sizes]
+    F = D * W * len(sizes)
+    for _ in range(2):
+        sims_frame(sims, sync=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sims_frame(sims, sync=False)
+    ctx.sync()
+    t_multi = (time.perf_counter() - t0) / steps
+    for s in sims:
+        s.frame(sync=False)
+        s.frame(sync=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sims:
+            s.frame(sync=False)
+    ctx.sync()
+    t_seq = (time.perf_counter() - t0) / steps
+    for s in sims:
+        s.close()
+    del sims
+    torch.cuda.empty_cache()
+    # plain per-wavelength launches: one direction, one field per simulation
+    t_plain = 0.0
+    for N in sizes:
+        one = [make(N, dirs[:1], wl["lambdas"][w:w + 1], scoeff[w:w + 1], [unwrap[w]]) for w in range(W)]
+        for s in one:
+            s.frame(sync=False)
+            s.frame(sync=False)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for s in one:
+                s.frame(sync=False)
+        ctx.sync()
+        t_plain += D * (time.perf_counter() - t0) / steps
+        for s in one:
+            s.close()
+    return {
+        "workload": "512^2 + 1024^2 + 2048^2, 16 directions (4 x 4 grid, +-1/3 and +-1 arcmin), 8 wavelengths, "
+                    "10 layers, 8192^2 masters shared",
+        "wavefront_frames_per_step": F,
+        "one_call": round(F / t_multi, 2),
+        "one_after_another": round(F / t_seq, 2),
+        "per_wavelength_plain": round(F / t_plain, 2),
+        "one_call_vs_per_wavelength": round(t_plain / t_multi, 2),
+        "unit": "wavefront-frames/s",
+        "steps": steps,
+        "how": "one_call: atsim_sims_frame (each size on its own stream, eager launches); one_after_another: "
+               "the three simulations' frames in sequence; per_wavelength_plain: one field per simulation, "
+               "timed on one direction per size and scaled to 16",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +358,8 @@ def main():
                          "memory (the PCIe-inclusive rate of a host hand-over; 0 = skip)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="frames of the per-wavelength comparison (one simulation per wavelength; 0 = skip)")
+    ap.add_argument("--mixed-steps", type=int, default=5,
+                    help="steps of the uneven-size workload (512/1024/2048, 16 directions; 0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -316,7 +402,7 @@ def main():
     tune = None
     if args.autotune:
         t0 = time.perf_counter()
-        tune = sim.autotune(frames=5)
+        tune = sim.autotune(frames=8)
         tune["search_s"] = round(time.perf_counter() - t0, 3)
     G = sim.gaps()
     F = wl["W"] * wl["D"]
@@ -407,6 +493,10 @@ def main():
         v1 = args.plain_steps * wl["W"] / el
         per_lambda = {"value": round(v1, 2), "unit": "wavefront-frames/s", "steps": args.plain_steps,
                       "how": "one simulation per wavelength, one field per launch, same masters and kernels"}
+
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = mixed_sizes(ctx, wl, sim, args.mixed_steps)
 
     N = wl["N"]
     kinds = ("row", "col", "row2", "col2", "phase")
@@ -529,6 +619,7 @@ def main():
         "pcie_inclusive": pcie,
         "per_wavelength_plain": per_lambda,
         "batched_vs_per_wavelength": round(value / per_lambda["value"], 2) if per_lambda else None,
+        "mixed_sizes": mixed,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c"),
                    "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),

@@ -230,6 +230,17 @@ int atsim_sim_normalize_masters(atsim_sim *sim, double r0_goal_px, const double 
  * [D][W][Nout][Nout] (phase in rad, unwrapped; amplitude); d_field (nullable)
  * receives the final complex fields [D][W][N][N]. */
 int atsim_sim_frame(atsim_sim *sim, float *d_pha, float *d_amp, void *d_field);
+/* One frame of n simulations of one context (any grid sizes): every
+ * simulation advances its layers and computes its frame as atsim_sim_frame
+ * does, into d_pha[k] / d_amp[k], each simulation on its own stream forked
+ * from and joined to the context stream, so that grids of different sizes
+ * run side by side (launched eagerly; no captured graph).  No counterpart in
+ * the reference (one process per grid). */
+int atsim_sims_frame(atsim_sim **sims, long n, float **d_pha, float **d_amp);
+/* Use the masters of `owner` (same context, M and L) instead of this
+ * simulation's own, which are freed: simulations of several grid sizes over
+ * one set of masters.  The owner must outlive this simulation. */
+int atsim_sim_use_masters(atsim_sim *sim, const atsim_sim *owner);
 /* Replace the current FP64 positions [D][L] [pixels]; the next frame
  * advances from them by v dt (the reference's REALTIME mode sets positions
  * from the clock, AT.c:2700-2704). */
@@ -261,8 +272,9 @@ int atsim_sim_block_sums(atsim_sim *sim, double *out, long n);
 /* Autotuner of the launch shapes for this simulation's sizes: times `frames`
  * frames (after two untimed ones) under each candidate of a coordinate
  * descent over the tuning keys gap_passes, streams, row_mw, col2_e, col_c
- * and col_e (two sweeps, values the kernels support at this grid size) and
- * leaves the fastest set on the context.  Frames are written to d_pha /
+ * and col_e (two sweeps, values the kernels support at this grid size; a
+ * candidate replaces the current set only when it is more than 1 % faster)
+ * and leaves the fastest set on the context.  Frames are written to d_pha /
  * d_amp (as atsim_sim_frame); the layer positions are restored afterwards,
  * so the series continues as if the search had not run.  report (nullable)
  * receives one line per candidate, "key=value ... ms=<per frame>", and a

@@ -1,0 +1,59 @@
+"""Frames of simulations of different grid sizes in one call (atsim_sims_frame)
+and simulations sharing one set of masters (atsim_sim_use_masters).
+
+The frames of 64^2, 128^2 and 256^2 simulations over the same masters, made
+together (eager first, then one captured graph replayed), are bit-identical
+to the same simulations run one by one, and the 64^2 one matches the FP64
+oracle at the bar of test_gpu_sim."""
+import numpy as np
+import pytest
+
+from tests.simcase import build_case, gpu_sim, oracle_sims
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _case(oracle, N):
+    return build_case(oracle, N=N, Nout=N - N // 8, M=512, alts=[0.0, 2500.0, 6000.0, 11000.0],
+                      cn2=[0.4, 0.3, 0.2, 0.1], speeds=[8.0, 14.0, 22.0, 31.0], dirs=[0.2, 1.1, 2.6, 4.0],
+                      lambdas=[0.5e-6, 1.0e-6, 1.6e-6], lam_ref=0.5e-6, dx=0.01, dt=0.001, r0_m=0.12, L0=20.0,
+                      l0=0.01, unwrap_mode=[0, 1, 1], directions=((0.0, 0.0), (1.5e-4, -2.9e-4)))
+
+
+def test_sims_frame_mixed_sizes(oracle):
+    from atmosphericturbulencesimul_amd.api import Context, sims_frame
+    sizes = (64, 128, 256)
+    cases = {N: _case(oracle, N) for N in sizes}
+    cx = Context(0)
+    cx.set_tuning("graph", 1)
+    try:
+        owner = gpu_sim(cx, cases[256])
+        multi = [gpu_sim(cx, cases[N]) for N in sizes[:2]] + [owner]
+        for s in multi[:2]:
+            s.use_masters(owner)
+        single = [gpu_sim(cx, cases[N]) for N in sizes]
+        for k in range(4):
+            print(f"multi-size frame {k}", flush=True)
+            outs = [(p.cpu().numpy(), a.cpu().numpy()) for p, a in sims_frame(multi)]
+            for s, (pm, am) in zip(single, outs):
+                p1, a1 = s.frame()
+                assert np.array_equal(pm, p1.cpu().numpy()) and np.array_equal(am, a1.cpu().numpy()), (s.N, k)
+        # a 64^2 simulation driven by atsim_sims_frame against the oracle
+        c = cases[64]
+        s64 = gpu_sim(cx, c)
+        s64.use_masters(owner)
+        osims = oracle_sims(oracle, c)
+        for k in range(3):
+            pg, ag = [t.cpu().numpy() for t in sims_frame([s64])[0]]
+            for d, os_ in enumerate(osims):
+                _, _, pha_o, amp_o = os_.frame(want_field=False)
+                for w in range(c.W):
+                    assert rel_l2(ag[d, w], amp_o[w]) < 1e-5
+                    bad = np.abs(pg[d, w] - pha_o[w]) > 1.0
+                    assert bad.sum() <= 1e-3 * bad.size
+                    assert rel_l2(pg[d, w][~bad], pha_o[w][~bad]) < 1e-5
+    finally:
+        cx.close()
+
