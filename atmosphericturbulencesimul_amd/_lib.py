@@ -48,6 +48,8 @@ _SIGS = {
     "atsim_permut": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long]),
     "atsim_fresnel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long,
                                 C.c_double, C.c_double, C.c_double]),
+    "atsim_fft2d_variant": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int]),
+    "atsim_row_fft_variant": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int]),
     "atsim_fft2d_nm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_int]),
     "atsim_fresnel_nm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long,
                                    C.c_double, C.c_double, C.c_double]),

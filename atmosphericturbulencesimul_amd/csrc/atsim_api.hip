@@ -370,6 +370,48 @@ int atsim_fft2d(atsim_ctx* c, void* d_data, long n, long nbatch, int sign) {
     return ATSIM_OK;
 }
 
+int atsim_fft2d_variant(atsim_ctx* c, void* d_data, long n, long nbatch, int sign, int variant) {
+    if (!c || !d_data) return ATSIM_EINVAL;
+    if (variant == 0) return atsim_fft2d(c, d_data, n, nbatch, sign);
+    if (variant != 1) return fail(c, ATSIM_EINVAL, "fft2d_variant: unknown variant %d", variant);
+    if (!fft_mfma_supported(n))
+        return fail(c, ATSIM_EUNSUPPORTED, "fft2d_variant: the matrix-core variant is built for n = 2048 (got %ld)", n);
+    if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fft2d_variant: nbatch %ld out of range", nbatch);
+    if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "fft2d_variant: sign must be -1 or +1");
+    float2* tw;
+    if (int r = get_twiddles(c, n, &tw)) return r;
+    float2* d = (float2*)d_data;
+    float2* tmp = nullptr;
+    HIPCHK(c, hipMallocAsync((void**)&tmp, sizeof(float2) * n * n * nbatch, c->stream));
+    int r = ATSIM_OK;
+    // rows, transpose, rows (the columns), transpose back
+    if (launch_row_fft_mfma(d, n * nbatch, sign, tw, c->stream) != hipSuccess ||
+        launch_transpose(d, tmp, n, nbatch, c->stream) != hipSuccess ||
+        launch_row_fft_mfma(tmp, n * nbatch, sign, tw, c->stream) != hipSuccess ||
+        launch_transpose(tmp, d, n, nbatch, c->stream) != hipSuccess)
+        r = fail(c, ATSIM_EHIP, "fft2d_variant: launch failed");
+    hipFreeAsync(tmp, c->stream);
+    return r;
+}
+
+int atsim_row_fft_variant(atsim_ctx* c, void* d_data, long n, long nrows, int sign, int variant) {
+    if (!c || !d_data) return ATSIM_EINVAL;
+    if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "row_fft_variant: sign must be -1 or +1");
+    if (nrows <= 0 || nrows % n) return fail(c, ATSIM_EINVAL, "row_fft_variant: nrows must be a multiple of n");
+    float2* tw;
+    if (int r = get_twiddles(c, n, &tw)) return r;
+    if (variant == 0) {
+        if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "row_fft_variant: size %ld unsupported", n);
+        ScreenArgs none{};
+        HIPCHK(c, launch_row(n, OP_FFT, sign, false, (float2*)d_data, nrows / n, nullptr, tw, none, c->stream));
+        return ATSIM_OK;
+    }
+    if (variant != 1 || !fft_mfma_supported(n))
+        return fail(c, ATSIM_EUNSUPPORTED, "row_fft_variant: variant %d at n = %ld is not built", variant, n);
+    HIPCHK(c, launch_row_fft_mfma((float2*)d_data, nrows, sign, tw, c->stream));
+    return ATSIM_OK;
+}
+
 int atsim_permut(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch) {
     if (!c || !d_data) return ATSIM_EINVAL;
     if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "permut: sizes must be > 0");

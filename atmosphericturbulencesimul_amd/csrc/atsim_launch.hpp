@@ -46,6 +46,13 @@ hipError_t launch_col(long n, int op, int dir, float2* data, long F, const float
                       const float2* tw, hipStream_t st, int c = 0, int e = 0, int wlo = 0, int wcnt = 0);
 
 hipError_t launch_permut(float2* data, long n0, long n1, long nbatch, hipStream_t st);
+// mixed-precision row transforms (atsim_fft_mfma.hip): length 2048, radix-16
+// stages on the matrix cores (FP16 hi + lo operands, FP32 accumulation);
+// nrows a multiple of 2
+bool fft_mfma_supported(long n);
+hipError_t launch_row_fft_mfma(float2* data, long nrows, int dir, const float2* tw, hipStream_t st);
+// out[b][x][y] = in[b][y][x], n a multiple of 32
+hipError_t launch_transpose(const float2* in, float2* out, long n, long nbatch, hipStream_t st);
 // transforms of any size: lines of length L (< = M) between strided images and
 // a padded scratch [nlines][M]; base of line l = (l / per_img) img_size +
 // (l % per_img) lstep, elements estride apart; chirp (nullable) multiplies

@@ -330,6 +330,53 @@ def mixed_sizes(ctx, wl, owner, steps):
     }
 
 
+def fft_variants(ctx, sim, reps=10):
+    """The 2048^2 2D FFT of the headline's fields (a frame's 8 final fields) in
+    the two arithmetic variants of atsim_fft2d_variant: 0 = FP32 Stockham,
+    1 = radix-16 stages as FP16 hi + lo DFT-16 products on the matrix cores.
+    Relative L2 error of each against numpy's FP64 FFT of the same fields,
+    and device time of the row transforms of all 8 x 2048 rows and of the
+    whole 2D transform (events on the library stream)."""
+    import ctypes as C
+    import torch
+    n = sim.N
+    field = torch.empty((sim.D, sim.W, n, n), dtype=torch.complex64, device="cuda")
+    sim.frame(field)
+    x = field.reshape(-1, n, n).contiguous()
+    nb = x.shape[0]
+    ref = np.fft.fft2(x.cpu().numpy().astype(np.complex128))
+    st = ctx.stream
+    out = {}
+    for variant in (0, 1):
+        y = x.clone()
+        torch.cuda.synchronize()
+        ctx._chk(ctx.L.atsim_fft2d_variant(ctx.h, C.c_void_p(y.data_ptr()), n, nb, -1, variant))
+        ctx.sync()
+        err = float(np.linalg.norm((y.cpu().numpy() - ref).ravel()) / np.linalg.norm(ref.ravel()))
+        y = x.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            ctx._chk(ctx.L.atsim_row_fft_variant(ctx.h, C.c_void_p(y.data_ptr()), n, n * nb, -1, variant))
+        e1.record(st)
+        e1.synchronize()
+        t_rows = e0.elapsed_time(e1) / reps
+        e0.record(st)
+        for _ in range(reps):
+            ctx._chk(ctx.L.atsim_fft2d_variant(ctx.h, C.c_void_p(y.data_ptr()), n, nb, -1, variant))
+        e1.record(st)
+        e1.synchronize()
+        t_2d = e0.elapsed_time(e1) / reps
+        name = "fp32_stockham" if variant == 0 else "fp16_hilo_mfma"
+        out[name] = {"rel_l2_vs_fp64": err, "row_pass_ms": round(t_rows, 4),
+                     "row_pass_gbs": round(16.0 * n * n * nb / (t_rows * 1e-3) / 1e9, 1),
+                     "fft2d_ms": round(t_2d, 4)}
+    out["case"] = (f"forward 2D FFT of the {nb} final 2048^2 fields of a headline frame; the FP16 variant's "
+                   "2D transform runs rows, transpose, rows, transpose")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +405,8 @@ def main():
                          "memory (the PCIe-inclusive rate of a host hand-over; 0 = skip)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="frames of the per-wavelength comparison (one simulation per wavelength; 0 = skip)")
+    ap.add_argument("--fft-variants", type=int, default=1, help="1: time and check the FP32 and FP16-MFMA 2D FFT "
+                    "variants on the headline's fields")
     ap.add_argument("--mixed-steps", type=int, default=5,
                     help="steps of the uneven-size workload (512/1024/2048, 16 directions; 0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -494,6 +543,7 @@ def main():
         per_lambda = {"value": round(v1, 2), "unit": "wavefront-frames/s", "steps": args.plain_steps,
                       "how": "one simulation per wavelength, one field per launch, same masters and kernels"}
 
+    variants = fft_variants(ctx, sim) if args.fft_variants else None
     mixed = None
     if args.mixed_steps > 0:
         mixed = mixed_sizes(ctx, wl, sim, args.mixed_steps)
@@ -620,6 +670,7 @@ def main():
         "per_wavelength_plain": per_lambda,
         "batched_vs_per_wavelength": round(value / per_lambda["value"], 2) if per_lambda else None,
         "mixed_sizes": mixed,
+        "fft_variants": variants,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c"),
                    "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),

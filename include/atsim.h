@@ -101,6 +101,17 @@ int atsim_device_ok(atsim_ctx *ctx);
  * sign = -1 forward, +1 backward: the reference's do2dfft / do2dffti
  * (src/fft/fft.c:1298-1315, FFTW plan over the swapped axes fft.c:1178-1190). */
 int atsim_fft2d(atsim_ctx *ctx, void *d_data, long n, long nbatch, int sign);
+/* The 2D transform of atsim_fft2d in one of two arithmetic variants:
+ * 0 = the FP32 Stockham transform (atsim_fft2d); 1 = radix-16 stages as
+ * DFT-16 matrix products on the matrix cores (v_mfma_f32_16x16x32_f16) with
+ * FP16 hi + lo operands (three products, FP32 accumulation, every vector
+ * scaled by a power of two first) and the radix-8 stage and twiddles in
+ * FP32, rows then columns through two transposes; variant 1 is built for
+ * n = 2048.  atsim_row_fft_variant transforms nrows contiguous rows of length
+ * n (nrows a multiple of n) in either variant.  No counterpart in the
+ * reference: the precision / speed comparison of the two paths. */
+int atsim_fft2d_variant(atsim_ctx *ctx, void *d_data, long n, long nbatch, int sign, int variant);
+int atsim_row_fft_variant(atsim_ctx *ctx, void *d_data, long n, long nrows, int sign, int variant);
 /* 2D c2c of any nbatch contiguous n0 x n1 images (n0 = x size, fastest):
  * the reference's FFT_do2dfft on any shape (FFTW plan over the swapped axes,
  * fft.c:1145-1209), unnormalised, sign as atsim_fft2d.  Square power-of-two
