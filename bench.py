@@ -377,6 +377,58 @@ def fft_variants(ctx, sim, reps=10):
     return out
 
 
+def shm_stream(frames, workdir):
+    """The 2048^2 headline streamed as the reference runs it: `atsim mkwfs 0.7 0`
+    (the science field at 0.7 um beside the 0.5 um reference field, 10
+    layers, 8192^2 masters) writing every science frame's OPD [m] and
+    amplitude to the shared-memory images <prefix>pha / <prefix>amp (with the
+    TIME keyword and the semaphores posted) and the per-time-span cubes in
+    host memory, 100 frames per time span.  Three runs in one directory: 100
+    frames (synthesises and caches the masters), 100 frames (cached masters),
+    `frames` frames; the streaming rate is (frames - 100) over the time the
+    third run takes beyond the second."""
+    import subprocess
+    cli = os.path.join(ROOT, "atmosphericturbulencesimul_amd", "atsim")
+    os.makedirs(workdir, exist_ok=True)
+    L = 10
+    alts = [20000.0 * k / 9 for k in range(L)]
+    prof = "# alt cn2 speed dir L0 l0 sigmaW Lwind\n" + "".join(
+        f"{a:.1f} {math.exp(-a / 2000.0):.6e} {5.0 + 35.0 * k / (L - 1):.3f} "
+        f"{(2.399963229728653 * k) % (2 * math.pi):.6f} 20.0 0.01 0 0\n" for k, a in enumerate(alts))
+    with open(os.path.join(workdir, "prof.txt"), "w") as f:
+        f.write(prof)
+
+    def conf(nspan):
+        return ("TURBULENCE_REF_WAVEL 0.5\nTURBULENCE_SEEING 0.859436\nTURBULENCE_PROF_FILE prof.txt\n"
+                "ZENITH_ANGLE 0.0\nWFOUTPUT 2\nSHM_OUTPUT 0\nMAKE_SWAVEFRONT 1\nSWF_WRITE2DISK 0\n"
+                "SHM_SOUTPUT 1\nSHM_SPREFIX atsimstream_\nSHM_SOUTPUTM 1\nPUPIL_SCALE 0.01\nWFsize 2048\n"
+                f"WFTIME_STEP 0.001\nTIME_SPAN 0.1\nNB_TSPAN {nspan}\nWF_RAW_SIZE 2048\nMASTER_SIZE 8192\n"
+                "WAVEFRONT_AMPLITUDE 1\nFRESNEL_PROPAGATION 1\nFRESNEL_PROPAGATION_BIN 1000.0\n"
+                "ATSIM_NOISE_SEED 19233\n")
+
+    env = dict(os.environ, ATSIM_SHM_DIR=workdir)
+    times = []
+    for nspan in (1, 1, frames // 100):
+        with open(os.path.join(workdir, "WFsim.conf"), "w") as f:
+            f.write(conf(nspan))
+        t0 = time.perf_counter()
+        p = subprocess.run([cli, "mkwfs", "0.7", "0"], cwd=workdir, env=env, capture_output=True, text=True,
+                           timeout=900)
+        times.append(time.perf_counter() - t0)
+        print(f"mkwfs {nspan * 100} frames: {times[-1]:.2f} s, rc {p.returncode}", file=sys.stderr, flush=True)
+        if p.returncode != 0:
+            raise RuntimeError("mkwfs failed: " + p.stdout[-2000:] + p.stderr[-2000:])
+    nfr = (frames // 100) * 100
+    rate = (nfr - 100) / (times[2] - times[1])
+    shm = os.path.join(workdir, "atsimstream_pha.im.shm")
+    return {"metric": "shared-memory streamed frames/s (2048^2, 10 layers, science 0.7 um + 0.5 um reference field)",
+            "value": round(rate, 2), "unit": "frames/s", "frames": nfr, "wall_s": [round(t, 2) for t in times],
+            "shm_image_bytes": os.path.getsize(shm) if os.path.exists(shm) else None,
+            "how": "atsim mkwfs 0.7 0 (compat frame loop: two fields per step, double-buffered device-to-host "
+                   "copies, shared-memory OPD and amplitude per frame, 100-frame host cubes); rate = extra "
+                   "frames over extra time between a 100-frame and the full run on cached masters"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +463,20 @@ def main():
                     help="steps of the uneven-size workload (512/1024/2048, 16 directions; 0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device-to-device copy of this size first (calibrates rocprof byte counters)")
+    ap.add_argument("--shm-stream", type=int, default=0,
+                    help="instead of the headline line: stream this many frames (a multiple of 100) of the "
+                         "2048^2 workload through atsim mkwfs to shared memory and print that rate")
+    ap.add_argument("--shm-dir", default="", help="working directory of --shm-stream (default: a temporary "
+                    "directory, removed afterwards; it holds 2.7 GB of cached masters)")
     args = ap.parse_args()
+    if args.shm_stream > 0:
+        import tempfile
+        if args.shm_dir:
+            print(json.dumps(shm_stream(args.shm_stream, os.path.abspath(args.shm_dir))))
+        else:
+            with tempfile.TemporaryDirectory(prefix="atsim_stream_") as d:
+                print(json.dumps(shm_stream(args.shm_stream, d)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the wavefront path is built and measured on a single MI355X "
               "(independent (lambda, direction) shards would split across GPUs with no exchange); "
