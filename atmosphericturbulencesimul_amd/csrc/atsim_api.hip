@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "atsim.h"
+#include "atsim_fft.hpp"
 #include "atsim_kernels.hpp"
 #include "atsim_launch.hpp"
 
@@ -75,18 +76,33 @@ static int fail(atsim_ctx* c, int code, const char* fmt, ...) {
 
 static bool pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// e^{-2 pi i m / n}, m < n, built in FP64
+// e^{-2 pi i m / n}, m < n, built in FP64, followed (powers of two) by the
+// per-stage tables of the transform plans: entries of the same table
+// gathered so that consecutive k are adjacent (tw_stage_off)
 static int get_twiddles(atsim_ctx* c, long n, float2** out) {
     auto it = c->twiddles.find(n);
     if (it != c->twiddles.end()) { *out = it->second; return 0; }
-    std::vector<float2> h(n);
+    const bool plans = pow2(n) && n >= 16 && n <= 8192;
+    const long extra = plans ? tw_tables_size((int)n) : 0;
+    std::vector<float2> h(n + extra);
     for (long m = 0; m < n; m++) {
         double a = -2.0 * M_PI * (double)m / (double)n;
         h[m] = make_float2((float)cos(a), (float)sin(a));
     }
+    if (plans) {
+        const int S = (ilog2c((int)n) + 3) / 4;
+        for (int dir = -1; dir <= 1; dir += 2)
+            for (int s = 1; s < S; s++) {
+                const int R = tw_radix((int)n, s, dir), NS = tw_ns((int)n, s, dir);
+                const long step = n / ((long)NS * R);
+                float2* o = h.data() + n + tw_stage_off((int)n, dir, s);
+                for (int e = 0; e < tw_terms(R); e++)
+                    for (int k = 0; k < NS; k++) o[e * NS + k] = h[((long)1 << e) * k * step];
+            }
+    }
     float2* d = nullptr;
-    HIPCHK(c, hipMalloc(&d, sizeof(float2) * n));
-    HIPCHK(c, copy_on_stream(c, d, h.data(), sizeof(float2) * n));
+    HIPCHK(c, hipMalloc(&d, sizeof(float2) * (n + extra)));
+    HIPCHK(c, copy_on_stream(c, d, h.data(), sizeof(float2) * (n + extra)));
     c->twiddles[n] = d;
     *out = d;
     return 0;

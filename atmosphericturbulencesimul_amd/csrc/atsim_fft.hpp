@@ -200,6 +200,70 @@ __device__ __forceinline__ void stage_compute(float2* v, int t, const float2* __
     }
 }
 
+// Per-stage twiddle tables.  Stage s >= 1 of a plan (radix R, NS = product of
+// the earlier radices, STEP = N / (NS R)) reads w^(2^e k STEP) for e with
+// 2^e < R and k = j mod NS.  Gathered into [e][k] order they are contiguous in
+// k, so that the lanes of a wave, whose k are consecutive, read whole cache
+// lines (the N-entry table read at stride 2^e STEP touches up to 2^e times
+// as many).  The tables follow the N entries of tw[]: forward stages first,
+// then inverse, each stage's [e][k] block in stage order.
+__host__ __device__ constexpr int tw_radix(int n, int s, int dir) {
+    const int S = (ilog2c(n) + 3) / 4, R0 = n >> (4 * (S - 1));
+    return dir < 0 ? (s == 0 ? R0 : 16) : (s == S - 1 ? R0 : 16);
+}
+__host__ __device__ constexpr int tw_ns(int n, int s, int dir) {
+    int p = 1;
+    for (int q = 0; q < s; q++) p *= tw_radix(n, q, dir);
+    return p;
+}
+__host__ __device__ constexpr int tw_terms(int r) { return r >= 16 ? 4 : r >= 8 ? 3 : r >= 4 ? 2 : 1; }
+__host__ __device__ constexpr int tw_stage_size(int n, int s, int dir) {
+    return s == 0 ? 0 : tw_terms(tw_radix(n, s, dir)) * tw_ns(n, s, dir);
+}
+// offset of stage s of direction dir in the tables (from tw + n)
+__host__ __device__ constexpr int tw_stage_off(int n, int dir, int s) {
+    const int S = (ilog2c(n) + 3) / 4;
+    int o = 0;
+    if (dir > 0)
+        for (int q = 0; q < S; q++) o += tw_stage_size(n, q, -1);
+    for (int q = 0; q < s; q++) o += tw_stage_size(n, q, dir);
+    return o;
+}
+__host__ __device__ constexpr int tw_tables_size(int n) { return tw_stage_off(n, +1, (ilog2c(n) + 3) / 4); }
+
+// stage_compute with the twiddles from the stage's table (tw: the N-entry
+// table followed by the per-stage tables); the same values, so the same
+// results
+template <int N, int R, int NS, int DIR, int E, int S>
+__device__ __forceinline__ void stage_compute_tab(float2* v, int t, const float2* __restrict__ tw) {
+    constexpr int T = N / E;
+    constexpr int OFF = N + tw_stage_off(N, DIR, S);
+#pragma unroll
+    for (int b = 0; b < E / R; b++) {
+        if constexpr (NS > 1) {
+            const int k = (t + b * T) % NS;
+            float2 p[4];  // w^1, w^2, w^4, w^8
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if ((1 << e) < R) p[e] = twiddle<N, DIR>(tw, OFF + e * NS + k);
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w = make_float2(1.f, 0.f);
+                bool first = true;
+#pragma unroll
+                for (int e = 3; e >= 0; e--)
+                    if (r & (1 << e)) {
+                        w = first ? p[e] : cmul(w, p[e]);
+                        first = false;
+                    }
+                v[b * R + r] = cmul(v[b * R + r], w);
+            }
+        }
+        dftR<R, DIR>(v + b * R);
+        if constexpr (E > 16) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // Padded LDS positions as one per-thread base plus compile-time offsets
 // (immediate offsets of the LDS instructions, no per-element address
 // registers).  For every plan here (all N, E = 16 or 32, every stage) the pad
@@ -239,9 +303,18 @@ __device__ __forceinline__ void stage_load(float2* v, const float2* lds, int t, 
         for (int r = 0; r < R; r++) v[b * R + r] = p[load_off<N, R, E>(b, r) * C];
 }
 
+// stage s >= 1 of a transform: twiddles from the N-entry table, or (TAB) from
+// the stage's own table that follows it
+template <int N, int R, int NS, int DIR, int E, int S, bool TAB>
+__device__ __forceinline__ void stage_compute_sel(float2* v, int t, const float2* __restrict__ tw) {
+    if constexpr (TAB) stage_compute_tab<N, R, NS, DIR, E, S>(v, t, tw);
+    else stage_compute<N, R, NS, DIR, E>(v, t, tw);
+}
+
 // Full transform on registers.  All threads of the workgroup must call it
-// (it contains __syncthreads); `lds` is this transform group's region.
-template <int N, int DIR, int C>
+// (it contains __syncthreads); `lds` is this transform group's region.  TAB:
+// tw carries the per-stage tables after its N entries.
+template <int N, int DIR, int C, bool TAB = false>
 __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, int c, const float2* __restrict__ tw) {
     using P = Plan<N>;
     stage_compute<N, P::radix(0, DIR), P::ns(0, DIR), DIR>(v, t, tw);
@@ -250,21 +323,21 @@ __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, int c, c
         stage_store<N, P::radix(0, DIR), P::ns(0, DIR), C>(v, lds, t, c);
         __syncthreads();
         stage_load<N, P::radix(1, DIR), C>(v, lds, t, c);
-        stage_compute<N, P::radix(1, DIR), P::ns(1, DIR), DIR>(v, t, tw);
+        stage_compute_sel<N, P::radix(1, DIR), P::ns(1, DIR), DIR, 16, 1, TAB>(v, t, tw);
     }
     if constexpr (P::S > 2) {
         __syncthreads();
         stage_store<N, P::radix(1, DIR), P::ns(1, DIR), C>(v, lds, t, c);
         __syncthreads();
         stage_load<N, P::radix(2, DIR), C>(v, lds, t, c);
-        stage_compute<N, P::radix(2, DIR), P::ns(2, DIR), DIR>(v, t, tw);
+        stage_compute_sel<N, P::radix(2, DIR), P::ns(2, DIR), DIR, 16, 2, TAB>(v, t, tw);
     }
     if constexpr (P::S > 3) {
         __syncthreads();
         stage_store<N, P::radix(2, DIR), P::ns(2, DIR), C>(v, lds, t, c);
         __syncthreads();
         stage_load<N, P::radix(3, DIR), C>(v, lds, t, c);
-        stage_compute<N, P::radix(3, DIR), P::ns(3, DIR), DIR>(v, t, tw);
+        stage_compute_sel<N, P::radix(3, DIR), P::ns(3, DIR), DIR, 16, 3, TAB>(v, t, tw);
     }
 }
 
@@ -315,21 +388,21 @@ __device__ __forceinline__ void exchange_split(float2* v, float* lds, int t, int
 
 // Full transform on registers, E elements per thread, exchanges through a
 // float LDS region of C interleaved transforms (C * Plan<N>::LDS floats).
-template <int N, int DIR, int C, int E>
+template <int N, int DIR, int C, int E, bool TAB = false>
 __device__ __forceinline__ void fft_regs_split(float2* v, float* lds, int t, int c, const float2* __restrict__ tw) {
     using P = Plan<N>;
     stage_compute<N, P::radix(0, DIR), P::ns(0, DIR), DIR, E>(v, t, tw);
     if constexpr (P::S > 1) {
         exchange_split<N, DIR, C, E, 0>(v, lds, t, c);
-        stage_compute<N, P::radix(1, DIR), P::ns(1, DIR), DIR, E>(v, t, tw);
+        stage_compute_sel<N, P::radix(1, DIR), P::ns(1, DIR), DIR, E, 1, TAB>(v, t, tw);
     }
     if constexpr (P::S > 2) {
         exchange_split<N, DIR, C, E, 1>(v, lds, t, c);
-        stage_compute<N, P::radix(2, DIR), P::ns(2, DIR), DIR, E>(v, t, tw);
+        stage_compute_sel<N, P::radix(2, DIR), P::ns(2, DIR), DIR, E, 2, TAB>(v, t, tw);
     }
     if constexpr (P::S > 3) {
         exchange_split<N, DIR, C, E, 2>(v, lds, t, c);
-        stage_compute<N, P::radix(3, DIR), P::ns(3, DIR), DIR, E>(v, t, tw);
+        stage_compute_sel<N, P::radix(3, DIR), P::ns(3, DIR), DIR, E, 3, TAB>(v, t, tw);
     }
 }
 

@@ -478,12 +478,18 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
     static_assert(P::first(-1) == P::last(+1), "the inverse plan must end in the forward plan's layout");
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int nf = gridDim.y;
-    // launch indices run fields-inner and each XCD takes a contiguous eighth
-    // of them: neighbouring column tiles of every field, and their phase
-    // plane tiles, are read through one XCD's L2 (266 us at 2048^2 x 8
-    // fields, against 285 us with 16-index chunks)
+    // launch indices run in groups of 4 adjacent column tiles (256 bytes of
+    // every row) of one field, fields next, and each XCD takes a contiguous
+    // eighth of them: the tiles that share 128-byte lines, and every field's
+    // phase-plane tiles, are read through one XCD's L2 at about the same
+    // time (2048^2 x 8 fields: 262 us and 1.19 x the algorithmic bytes,
+    // against 266 us and 1.27 x with single tiles and 285 us with 16-index
+    // chunks)
+    constexpr int TG = 4;
+    static_assert((N / C) % TG == 0, "column tiles come in groups of 4");
     const int lin = xcd_slab(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int ct = lin / nf, fi = lin - ct * nf;
+    const int h = lin % TG, rest = lin / TG;
+    const int fi = rest % nf, ct = (rest / nf) * TG + h;
     const int nw = sa.wcnt > 0 ? sa.wcnt : sa.W;
     const int d = fi / nw, w = (sa.wcnt > 0 ? sa.wlo : 0) + (fi - d * nw);
     const int f = d * sa.W + w;
@@ -494,13 +500,13 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
     float2 v[E];
 #pragma unroll
     for (int q = 0; q < E; q++) v[q] = buf_load2(fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
-    fft_regs_split<N, -1, C, E>(v, lds, t, c, tw);
+    fft_regs_split<N, -1, C, E, true>(v, lds, t, c, tw);
     {
         const auto hr = buf_rsrc(tfp + (size_t)w * N, (uint32_t)N * 8);
 #pragma unroll
         for (int q = 0; q < E; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t * 8, reg_index<N, RL, E>(0, q) * 8));
     }
-    fft_regs_split<N, +1, C, E>(v, lds, t, c, tw);
+    fft_regs_split<N, +1, C, E, true>(v, lds, t, c, tw);
     // the group's phase sum_l s_wl V_l, 8 register slots at a time
 #pragma unroll
     for (int q0 = 0; q0 < E; q0 += 8) {
@@ -520,13 +526,13 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
         }
     }
     const int t2 = opaque(t), c2 = opaque(c);
-    fft_regs_split<N, -1, C, E>(v, lds, t2, c2, tw);
+    fft_regs_split<N, -1, C, E, true>(v, lds, t2, c2, tw);
     {
         const auto hr = buf_rsrc(tfc + (size_t)w * N, (uint32_t)N * 8);
 #pragma unroll
         for (int q = 0; q < E; q++) v[q] = cmul(v[q], buf_load2(hr, (uint32_t)t2 * 8, reg_index<N, RL, E>(0, q) * 8));
     }
-    fft_regs_split<N, +1, C, E>(v, lds, t2, c2, tw);
+    fft_regs_split<N, +1, C, E, true>(v, lds, t2, c2, tw);
 #pragma unroll
     for (int q = 0; q < E; q++) buf_store2(v[q], fr, lo, (reg_index<N, RI, E>(0, q) * N) * 8);
 }
@@ -636,11 +642,11 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     static_assert(RI == RO, "the inverse plan must end in the layout the forward plan starts from");
     if constexpr (PRE) {
         float2* Lp = lds + lw * P::LDS;
-        fft_regs<N, -1, 1>(v, Lp, t, 0, tw);
+        fft_regs<N, -1, 1, true>(v, Lp, t, 0, tw);
         const float2* hp = tfp + (size_t)w * N;
 #pragma unroll
         for (int q = 0; q < 16; q++) v[q] = cmul(v[q], hp[reg_index<N, RL>(t, q)]);
-        fft_regs<N, +1, 1>(v, Lp, t, 0, tw);
+        fft_regs<N, +1, 1, true>(v, Lp, t, 0, tw);
     }
     __shared__ LayerSample sls[MAX_LAYERS];
     __shared__ double ss[8][MAX_LAYERS];
@@ -686,11 +692,11 @@ k_row_fresnel(float2* __restrict__ data, const float2* __restrict__ tf, const fl
     }
     float2* Lr = lds + lw * P::LDS;
     const int t2 = PRE ? opaque(t) : t;  // with PRE: indices recomputed, not kept live through the screen
-    fft_regs<N, -1, 1>(v, Lr, t2, 0, tw);
+    fft_regs<N, -1, 1, true>(v, Lr, t2, 0, tw);
     const float2* h = tf + (size_t)w * N;
 #pragma unroll
     for (int q = 0; q < 16; q++) v[q] = cmul(v[q], h[reg_index<N, RL>(t2, q)]);
-    fft_regs<N, +1, 1>(v, Lr, t2, 0, tw);
+    fft_regs<N, +1, 1, true>(v, Lr, t2, 0, tw);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 16; q++) rowp[reg_index<N, RO>(t, q)] = v[q];

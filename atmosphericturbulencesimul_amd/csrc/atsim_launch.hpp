@@ -22,7 +22,8 @@ hipError_t launch_row(long n, int op, int dir, bool gen, float2* data, long F, c
 // and direction); gen: the incoming field is 1.  Tables tf + w n.
 // mw: wavelengths per workgroup (0 = largest that fits).  tfp (not with
 // gen): the x factor of the gap before the group is applied first (tables
-// tfp + w n).
+// tfp + w n).  tw: the n twiddles followed by the per-stage tables
+// (tw_tables_size(n) entries, tw_stage_off).
 hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float2* tf, const float2* tw,
                               const ScreenArgs& sa, hipStream_t st, int mw = 0, const float2* tfp = nullptr);
 // Column pass around a layer group: y factor tfp of the gap before the
@@ -30,7 +31,8 @@ hipError_t launch_row_fresnel(long n, bool gen, float2* data, int D, const float
 // sampled phases in planes slot0 .. of vq), y factor tf of the gap after it;
 // every field of the wavelength range of sa (tables + w n).  n <= 4096.
 // e: elements per thread (0 = default: 32 with float exchanges for
-// 256 <= n <= 2048, else 16; 16 forces 16)
+// 256 <= n <= 2048, else 16; 16 forces 16).  tw as for launch_row_fresnel
+// (the 32-element passes read the per-stage tables).
 hipError_t launch_col_screen(long n, float2* data, int D, const float2* tfp, const float2* tf, const float2* tw,
                              const ScreenArgs& sa, const double* vq, int slot0, hipStream_t st, int e = 0);
 // Sampled phase planes (FP64 quarter turns) of layers slot_layer[0 ..
