@@ -149,12 +149,23 @@ struct Plan {
     static constexpr int first(int dir) { return radix(0, dir); }
     static constexpr int last(int dir) { return radix(S - 1, dir); }
     static constexpr int LDS = N + N / 16;                     // padded LDS elements per transform
+    // the same for C > 1 interleaved transforms (xpad: N / 8 pad slots when
+    // the forward plan starts with radix 8)
+    static constexpr int LDSC = R0 == 8 ? N + N / 8 : N + N / 16;
 };
 
-// LDS addressing: element idx of transform lane `c` in a group of C interleaved
-// transforms; one pad slot every 16 elements.
-template <int C>
-__device__ __forceinline__ int lds_addr(int idx, int c) { return (idx + (idx >> 4)) * C + c; }
+// Pad shift of exchange S (after stage S) of a transform in direction DIR
+// whose LDS region interleaves C transforms: one pad slot every 2^shift
+// elements.  The gfx950 LDS has 32 four-byte banks and serves 32 lanes (b32)
+// or 16 lanes (b64) per cycle; with C = 8 those lanes are 8 transforms x 4
+// (or 2) consecutive threads, so their positions must differ mod 4 (mod 2).
+// The stores of a forward plan's first radix-8 stage are 8 elements apart:
+// a pad every 8 elements separates them (every 16 put two threads on one
+// bank); every other exchange is conflict-free with a pad every 16.
+template <int N, int DIR, int C, int S>
+__device__ __forceinline__ constexpr int xpad() {
+    return (C > 1 && DIR < 0 && S == 0 && Plan<N>::R0 == 8) ? 3 : 4;
+}
 
 template <int N, int DIR>
 __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
@@ -267,43 +278,44 @@ __device__ __forceinline__ void stage_compute_tab(float2* v, int t, const float2
     }
 }
 
-// Padded LDS positions as one per-thread base plus compile-time offsets
-// (immediate offsets of the LDS instructions, no per-element address
-// registers).  For every plan here (all N, E = 16 or 32, every stage) the pad
-// of a store position P0 + r NS is pad(P0) + r NS / 16, and the pad of a
-// load position t + m (m = b T + r N / R) is pad(t) + m / 16: the carries
-// never cross a 16-element boundary.
-template <int N, int R, int NS, int E>
+// Padded LDS positions (one pad slot every 2^PS elements) as one per-thread
+// base plus compile-time offsets (immediate offsets of the LDS instructions,
+// no per-element address registers).  For every plan here (all N, E = 16 or
+// 32, every stage, PS = 4, and PS = 3 after a first radix-8 stage) the pad
+// of a store position P0 + r NS is pad(P0) + pad(r NS), and the pad of a
+// load position t + m (m = b T + r N / R) is pad(t) + pad(m): the carries
+// never cross a 2^PS-element boundary.
+template <int N, int R, int NS, int E, int PS = 4>
 __device__ __forceinline__ int store_base(int t, int b) {
     constexpr int T = N / E;
     const int j = t + b * T;
     const int p0 = (j / NS) * NS * R + j % NS;
-    return p0 + (p0 >> 4);
+    return p0 + (p0 >> PS);
 }
-template <int R, int NS>
-__device__ __forceinline__ constexpr int store_off(int r) { return r * NS + (r * NS) / 16; }
-template <int N, int R, int E>
+template <int R, int NS, int PS = 4>
+__device__ __forceinline__ constexpr int store_off(int r) { return r * NS + ((r * NS) >> PS); }
+template <int N, int R, int E, int PS = 4>
 __device__ __forceinline__ constexpr int load_off(int b, int r) {
-    return (b * (N / E) + r * (N / R)) + (b * (N / E) + r * (N / R)) / 16;
+    return (b * (N / E) + r * (N / R)) + ((b * (N / E) + r * (N / R)) >> PS);
 }
 
-template <int N, int R, int NS, int C, int E = 16>
+template <int N, int R, int NS, int C, int E = 16, int PS = 4>
 __device__ __forceinline__ void stage_store(const float2* v, float2* lds, int t, int c) {
 #pragma unroll
     for (int b = 0; b < E / R; b++) {
-        float2* p = lds + store_base<N, R, NS, E>(t, b) * C + c;
+        float2* p = lds + store_base<N, R, NS, E, PS>(t, b) * C + c;
 #pragma unroll
-        for (int r = 0; r < R; r++) p[store_off<R, NS>(r) * C] = v[b * R + r];
+        for (int r = 0; r < R; r++) p[store_off<R, NS, PS>(r) * C] = v[b * R + r];
     }
 }
 
-template <int N, int R, int C, int E = 16>
+template <int N, int R, int C, int E = 16, int PS = 4>
 __device__ __forceinline__ void stage_load(float2* v, const float2* lds, int t, int c) {
-    const float2* p = lds + (t + (t >> 4)) * C + c;
+    const float2* p = lds + (t + (t >> PS)) * C + c;
 #pragma unroll
     for (int b = 0; b < E / R; b++)
 #pragma unroll
-        for (int r = 0; r < R; r++) v[b * R + r] = p[load_off<N, R, E>(b, r) * C];
+        for (int r = 0; r < R; r++) v[b * R + r] = p[load_off<N, R, E, PS>(b, r) * C];
 }
 
 // stage s >= 1 of a transform: twiddles from the N-entry table, or (TAB) from
@@ -323,23 +335,23 @@ __device__ __forceinline__ void fft_regs(float2* v, float2* lds, int t, int c, c
     stage_compute<N, P::radix(0, DIR), P::ns(0, DIR), DIR>(v, t, tw);
     if constexpr (P::S > 1) {
         __syncthreads();
-        stage_store<N, P::radix(0, DIR), P::ns(0, DIR), C>(v, lds, t, c);
+        stage_store<N, P::radix(0, DIR), P::ns(0, DIR), C, 16, xpad<N, DIR, C, 0>()>(v, lds, t, c);
         __syncthreads();
-        stage_load<N, P::radix(1, DIR), C>(v, lds, t, c);
+        stage_load<N, P::radix(1, DIR), C, 16, xpad<N, DIR, C, 0>()>(v, lds, t, c);
         stage_compute_sel<N, P::radix(1, DIR), P::ns(1, DIR), DIR, 16, 1, TAB>(v, t, tw);
     }
     if constexpr (P::S > 2) {
         __syncthreads();
-        stage_store<N, P::radix(1, DIR), P::ns(1, DIR), C>(v, lds, t, c);
+        stage_store<N, P::radix(1, DIR), P::ns(1, DIR), C, 16, xpad<N, DIR, C, 1>()>(v, lds, t, c);
         __syncthreads();
-        stage_load<N, P::radix(2, DIR), C>(v, lds, t, c);
+        stage_load<N, P::radix(2, DIR), C, 16, xpad<N, DIR, C, 1>()>(v, lds, t, c);
         stage_compute_sel<N, P::radix(2, DIR), P::ns(2, DIR), DIR, 16, 2, TAB>(v, t, tw);
     }
     if constexpr (P::S > 3) {
         __syncthreads();
-        stage_store<N, P::radix(2, DIR), P::ns(2, DIR), C>(v, lds, t, c);
+        stage_store<N, P::radix(2, DIR), P::ns(2, DIR), C, 16, xpad<N, DIR, C, 2>()>(v, lds, t, c);
         __syncthreads();
-        stage_load<N, P::radix(3, DIR), C>(v, lds, t, c);
+        stage_load<N, P::radix(3, DIR), C, 16, xpad<N, DIR, C, 2>()>(v, lds, t, c);
         stage_compute_sel<N, P::radix(3, DIR), P::ns(3, DIR), DIR, 16, 3, TAB>(v, t, tw);
     }
 }
@@ -353,24 +365,24 @@ __device__ __forceinline__ int reg_index(int t, int q) {
 // The same exchange through an LDS array of floats, one component at a time
 // (real parts, then imaginary parts): half the LDS of a float2 exchange for
 // two more barriers per stage.
-template <int N, int R, int NS, int C, int E, int PART>
+template <int N, int R, int NS, int C, int E, int PART, int PS = 4>
 __device__ __forceinline__ void stage_store_part(const float2* v, float* lds, int t, int c) {
 #pragma unroll
     for (int b = 0; b < E / R; b++) {
-        float* p = lds + store_base<N, R, NS, E>(t, b) * C + c;
+        float* p = lds + store_base<N, R, NS, E, PS>(t, b) * C + c;
 #pragma unroll
-        for (int r = 0; r < R; r++) p[store_off<R, NS>(r) * C] = PART ? v[b * R + r].y : v[b * R + r].x;
+        for (int r = 0; r < R; r++) p[store_off<R, NS, PS>(r) * C] = PART ? v[b * R + r].y : v[b * R + r].x;
     }
 }
 
-template <int N, int R, int C, int E, int PART>
+template <int N, int R, int C, int E, int PART, int PS = 4>
 __device__ __forceinline__ void stage_load_part(float2* v, const float* lds, int t, int c) {
-    const float* p = lds + (t + (t >> 4)) * C + c;
+    const float* p = lds + (t + (t >> PS)) * C + c;
 #pragma unroll
     for (int b = 0; b < E / R; b++)
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const float x = p[load_off<N, R, E>(b, r) * C];
+            const float x = p[load_off<N, R, E, PS>(b, r) * C];
             if (PART) v[b * R + r].y = x;
             else v[b * R + r].x = x;
         }
@@ -379,14 +391,15 @@ __device__ __forceinline__ void stage_load_part(float2* v, const float* lds, int
 template <int N, int DIR, int C, int E, int S>
 __device__ __forceinline__ void exchange_split(float2* v, float* lds, int t, int c) {
     using P = Plan<N>;
+    constexpr int PS = xpad<N, DIR, C, S>();
     __syncthreads();
-    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 0>(v, lds, t, c);
+    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 0, PS>(v, lds, t, c);
     __syncthreads();
-    stage_load_part<N, P::radix(S + 1, DIR), C, E, 0>(v, lds, t, c);
+    stage_load_part<N, P::radix(S + 1, DIR), C, E, 0, PS>(v, lds, t, c);
     __syncthreads();
-    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 1>(v, lds, t, c);
+    stage_store_part<N, P::radix(S, DIR), P::ns(S, DIR), C, E, 1, PS>(v, lds, t, c);
     __syncthreads();
-    stage_load_part<N, P::radix(S + 1, DIR), C, E, 1>(v, lds, t, c);
+    stage_load_part<N, P::radix(S + 1, DIR), C, E, 1, PS>(v, lds, t, c);
 }
 
 // Full transform on registers, E elements per thread, exchanges through a

@@ -333,7 +333,7 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
     using P = Plan<N>;
     constexpr int RI = (OP == OP_FRESNEL) ? P::first(-1) : P::first(DIR);
     constexpr int RO = (OP == OP_FRESNEL) ? P::last(+1) : P::last(DIR);
-    __shared__ float2 lds[C * P::LDS];
+    __shared__ float2 lds[C * P::LDSC];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int ntile = N / C;
     const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
@@ -361,8 +361,8 @@ k_col(float2* __restrict__ data, const float2* __restrict__ tf, int W, const flo
 
 // Column Fresnel pass with E = 32 elements per thread (N / 32 threads per
 // column, C * N / 32 per workgroup) and float exchanges: C columns take
-// C * (N + N/16) * 4 B of LDS.  C = 8 (68 KB at N = 2048) lets two
-// workgroups share a CU; C = 16 (136 KB) reads 128-byte row segments.  At
+// C * Plan::LDSC * 4 B of LDS.  C = 8 (72 KB at N = 2048) lets two
+// workgroups share a CU; C = 16 (144 KB) reads 128-byte row segments.  At
 // most 128 VGPRs (4 waves per SIMD).
 template <int N, int C, int E>
 __global__ void __launch_bounds__(C * (N / E)) __attribute__((amdgpu_waves_per_eu(4)))
@@ -370,7 +370,7 @@ k_col_split(float2* __restrict__ data, const float2* __restrict__ tf, int W, con
             int wcnt) {
     using P = Plan<N>;
     constexpr int RI = P::first(-1), RO = P::last(+1), RL = P::last(-1);
-    __shared__ float lds[C * P::LDS];
+    __shared__ float lds[C * P::LDSC];
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int ntile = N / C;
     const int tile = xcd_swizzle(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
@@ -407,7 +407,7 @@ k_col_screen(float2* __restrict__ data, const float2* __restrict__ tfp, const fl
              const float2* __restrict__ tw, ScreenArgs sa, const double* __restrict__ vq, int slot0) {
     using P = Plan<N>;
     constexpr int RI = P::first(-1), RL = P::last(-1);
-    __shared__ float2 lds[C * P::LDS];
+    __shared__ float2 lds[C * P::LDSC];
     static_assert(P::first(-1) == P::last(+1), "the inverse plan must end in the forward plan's layout");
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int nf = gridDim.y;
@@ -464,8 +464,8 @@ k_col_screen(float2* __restrict__ data, const float2* __restrict__ tfp, const fl
 }
 
 // The same column pass with E = 32 elements per thread (N / 32 threads per
-// column) exchanging through float LDS: C = 8 columns take 8 (N + N/16) 4 B
-// (68 KB at N = 2048), so two workgroups share a CU and one's barriers and
+// column) exchanging through float LDS: C = 8 columns take 8 Plan::LDSC 4 B
+// (72 KB at N = 2048), so two workgroups share a CU and one's barriers and
 // loads overlap the other's arithmetic.  At most 128 VGPRs (4 waves per
 // SIMD).
 template <int N, int C, int E>
@@ -474,7 +474,7 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
                    const float2* __restrict__ tw, ScreenArgs sa, const double* __restrict__ vq, int slot0) {
     using P = Plan<N>;
     constexpr int RI = P::first(-1), RL = P::last(-1);
-    __shared__ float lds[C * P::LDS];
+    __shared__ float lds[C * P::LDSC];
     static_assert(P::first(-1) == P::last(+1), "the inverse plan must end in the forward plan's layout");
     const int c = threadIdx.x % C, t = threadIdx.x / C;
     const int nf = gridDim.y;
@@ -743,10 +743,10 @@ static hipError_t row_dispatch(int op, int dir, bool gen, float2* data, long F, 
     return hipGetLastError();
 }
 
-// column tile widths available for n (C * n/16 <= 1024 threads, C * LDS <= 139 KB)
+// column tile widths available for n (C * n/16 <= 1024 threads, C * LDSC <= 144 KB)
 template <int N, int C>
 constexpr bool col_ok() {
-    return C <= N && C * (N / 16) <= 1024 && (long)C * Plan<N>::LDS * 8 <= 139264;
+    return C <= N && C * (N / 16) <= 1024 && (long)C * Plan<N>::LDSC * 8 <= 147456;
 }
 
 template <int N, int C>
