@@ -835,7 +835,7 @@ struct atsim_sim {
     // extra streams of the multi-stream Fresnel passes (set k > 0 on xs_stream[k - 1])
     static constexpr int MAX_SETS = 4;
     hipStream_t xs_stream[MAX_SETS - 1] = {};
-    hipEvent_t ev_fork = nullptr, ev_first[MAX_SETS - 1] = {}, ev_join[MAX_SETS - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_phase = nullptr, ev_first[MAX_SETS - 1] = {}, ev_join[MAX_SETS - 1] = {};
     float* dres = nullptr;   // [F][Nout][Nout] block-rule remainders
     int* blk = nullptr;      // [Nout] block index of each output column / row
     double* bsum = nullptr;
@@ -950,6 +950,7 @@ static void sim_free(atsim_sim* s) {
         if (s->ev_join[k]) hipEventDestroy(s->ev_join[k]);
     }
     if (s->ev_fork) hipEventDestroy(s->ev_fork);
+    if (s->ev_phase) hipEventDestroy(s->ev_phase);
     hipFree(s->blk);
     hipFree(s->bsum);
     hipFree(s->bin);
@@ -1032,7 +1033,8 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     SIMALLOC(s->blk, sizeof(int) * (s->Nout > 0 ? s->Nout : 1));
     if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
-    if (r == ATSIM_OK && hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess)
+    if (r == ATSIM_OK && (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                          hipEventCreateWithFlags(&s->ev_phase, hipEventDisableTiming) != hipSuccess))
         r = fail(c, ATSIM_EHIP, "sim: event creation failed");
     for (int k = 0; r == ATSIM_OK && k < atsim_sim::MAX_SETS - 1; k++)
         if (hipStreamCreateWithFlags(&s->xs_stream[k], hipStreamNonBlocking) != hipSuccess ||
@@ -1271,15 +1273,20 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         // k - 1's first pass, so that one set's row pass (arithmetic-bound)
         // runs beside another's column pass (memory-bound).  All join the
         // context stream before extraction.  The phase planes of the column
-        // passes are sampled first, on the context stream.
-        if (fused) HIPCHK(c, phase(ms));
+        // passes are sampled on set 1's stream beside set 0's first (row)
+        // pass; every other set's first column pass waits for them.
         HIPCHK(c, hipEventRecord(s->ev_fork, ms));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
+        if (fused) {
+            HIPCHK(c, phase(s->xs_stream[0]));
+            HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
+        }
         for (long step = 0; step < nsteps; step++) {
             for (int k = 0; k < nsets; k++) {
                 hipStream_t st = k ? s->xs_stream[k - 1] : ms;
                 sa.wlo = (int)(W * k / nsets);
                 sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
+                if (fused && step == 1 && k != 1) HIPCHK(c, hipStreamWaitEvent(st, s->ev_phase, 0));
                 HIPCHK(c, launch_step(s, fused, step, sa, st));
                 if (step == 0 && k + 1 < nsets) {
                     HIPCHK(c, hipEventRecord(s->ev_first[k], st));
