@@ -1780,6 +1780,8 @@ int atsim_gemm_bf16_ex(atsim_ctx* c, long M, long N, long K, long batch, float a
                        long sam, long sak, const float* B, long sbb, long sbk, long sbn, float beta, float* C,
                        long scb, long scm, long scn, const float* bias, float* pre, int gelu) {
     if (!c || !A || !B || !C || M < 0 || N < 0 || K < 0 || batch < 0) return ATSIM_EINVAL;
+    if (sab < 0 || sam < 0 || sak < 0 || sbb < 0 || sbk < 0 || sbn < 0 || scb < 0 || scm < 0 || scn < 0)
+        return fail(c, ATSIM_EINVAL, "gemm: negative stride");
     GemmArgs g{M, N, K, batch, alpha, beta, A, sab, sam, sak, B, sbb, sbk, sbn, C, scb, scm, scn, bias, pre,
                gelu ? 1 : 0, 0, 0};
     hipError_t e = launch_gemm_bf16(g, c->stream);

@@ -333,7 +333,11 @@ int atsim_sim_layer_screens(atsim_sim *sim, float *d_out);
  * A_b (M x K) at A + b sab + m sam + k sak, B_b (K x N) at B + b sbb + k sbk
  * + n sbn, C_b (M x N) at C + b scb + m scm + n scn (element strides).  The
  * operands are rounded to BF16 and multiplied on the matrix cores with FP32
- * accumulation.  batch and ceil(M / 64) at most 65535. */
+ * accumulation.  batch and ceil(M / 64) at most 65535.  Strides are
+ * non-negative (ATSIM_EINVAL otherwise) and otherwise trusted: the library
+ * does not know the allocations, so the caller guarantees that every
+ * addressed element of A, B, C (and bias, pre) lies inside its buffer (the
+ * Python front end, fno.py, checks its operands against their tensors). */
 int atsim_gemm_bf16(atsim_ctx *ctx, long M, long N, long K, long batch, float alpha,
                     const float *A, long sab, long sam, long sak,
                     const float *B, long sbb, long sbk, long sbn, float beta,
