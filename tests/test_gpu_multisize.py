@@ -2,9 +2,9 @@
 and simulations sharing one set of masters (atsim_sim_use_masters).
 
 The frames of 64^2, 128^2 and 256^2 simulations over the same masters, made
-together (eager first, then one captured graph replayed), are bit-identical
-to the same simulations run one by one, and the 64^2 one matches the FP64
-oracle at the bar of test_gpu_sim."""
+together (each on its own stream, eager launches), are bit-identical to the
+same simulations run one by one (graph-replayed frames), and the 64^2 one
+matches the FP64 oracle at the bar of test_gpu_sim."""
 import numpy as np
 import pytest
 
