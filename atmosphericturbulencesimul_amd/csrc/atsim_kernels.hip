@@ -1698,6 +1698,9 @@ hipError_t launch_block_counts(const double* bin, float* ch, unsigned* cmax, con
 // pha = atan2 + 2 pi (n + count((r - peak) / 2 pi)) with the block's peak
 // (AT.c:4240-4262: count((PHI - atan2 - peak) / 2 pi), the whole turns n of
 // PHI - atan2 already in pha).  One output row of one field per workgroup.
+// Only the remainder map is read in full: pha is read and written only where
+// a count is not zero (a zero count leaves pha's bits unchanged: fma(2 pi,
+// +0, p) = p), which for a block's pixels around its peak is rare.
 __global__ void __launch_bounds__(256) k_block_apply(float* __restrict__ pha, const float* __restrict__ dres,
                                                      const double* __restrict__ bin, const int* __restrict__ flist,
                                                      const int* __restrict__ blk, int Nout, int xs) {
@@ -1714,20 +1717,27 @@ __global__ void __launch_bounds__(256) k_block_apply(float* __restrict__ pha, co
         for (int i4 = threadIdx.x; i4 < Nout / 4; i4 += 256) {
             const int4 b = reinterpret_cast<const int4*>(blk)[i4];
             const float4 r = reinterpret_cast<const float4*>(rrow)[i4];
-            float4 p = reinterpret_cast<float4*>(prow)[i4];
             const double pk0 = xs > 0 ? brow[b.x] : 0.0, pk1 = xs > 0 ? brow[b.y] : 0.0;
             const double pk2 = xs > 0 ? brow[b.z] : 0.0, pk3 = xs > 0 ? brow[b.w] : 0.0;
-            p.x = (float)fma(2.0 * M_PI, unwrap_count(((double)r.x - pk0) * inv2pi), (double)p.x);
-            p.y = (float)fma(2.0 * M_PI, unwrap_count(((double)r.y - pk1) * inv2pi), (double)p.y);
-            p.z = (float)fma(2.0 * M_PI, unwrap_count(((double)r.z - pk2) * inv2pi), (double)p.z);
-            p.w = (float)fma(2.0 * M_PI, unwrap_count(((double)r.w - pk3) * inv2pi), (double)p.w);
-            reinterpret_cast<float4*>(prow)[i4] = p;
+            const double n0 = unwrap_count(((double)r.x - pk0) * inv2pi);
+            const double n1 = unwrap_count(((double)r.y - pk1) * inv2pi);
+            const double n2 = unwrap_count(((double)r.z - pk2) * inv2pi);
+            const double n3 = unwrap_count(((double)r.w - pk3) * inv2pi);
+            if (n0 != 0.0 || n1 != 0.0 || n2 != 0.0 || n3 != 0.0) {
+                float4 p = reinterpret_cast<float4*>(prow)[i4];
+                p.x = (float)fma(2.0 * M_PI, n0, (double)p.x);
+                p.y = (float)fma(2.0 * M_PI, n1, (double)p.y);
+                p.z = (float)fma(2.0 * M_PI, n2, (double)p.z);
+                p.w = (float)fma(2.0 * M_PI, n3, (double)p.w);
+                reinterpret_cast<float4*>(prow)[i4] = p;
+            }
         }
         return;
     }
     for (int ii = threadIdx.x; ii < Nout; ii += 256) {
         const double peak = xs > 0 ? brow[blk[ii]] : 0.0;
-        prow[ii] = (float)fma(2.0 * M_PI, unwrap_count(((double)rrow[ii] - peak) * inv2pi), (double)prow[ii]);
+        const double n = unwrap_count(((double)rrow[ii] - peak) * inv2pi);
+        if (n != 0.0) prow[ii] = (float)fma(2.0 * M_PI, n, (double)prow[ii]);
     }
 }
 
