@@ -1697,54 +1697,68 @@ hipError_t launch_block_counts(const double* bin, float* ch, unsigned* cmax, con
 
 // pha = atan2 + 2 pi (n + count((r - peak) / 2 pi)) with the block's peak
 // (AT.c:4240-4262: count((PHI - atan2 - peak) / 2 pi), the whole turns n of
-// PHI - atan2 already in pha).  One output row of one field per workgroup.
-// Only the remainder map is read in full: pha is read and written only where
-// a count is not zero (a zero count leaves pha's bits unchanged: fma(2 pi,
-// +0, p) = p), which for a block's pixels around its peak is rare.
+// PHI - atan2 already in pha).  APPLY_ROWS output rows of one field per
+// workgroup, their remainder loads issued together.  Only the remainder map
+// is read in full: pha is read and written only where a count is not zero
+// (a zero count leaves pha's bits unchanged: fma(2 pi, +0, p) = p), which
+// for a block's pixels around its peak is rare.
+constexpr int APPLY_ROWS = 4;
+__device__ __forceinline__ double apply_count(float r, double peak) {
+    // r is a float-rounded remainder, so (r - peak) / (2 pi) as a product with
+    // 1 / (2 pi) changes no decision that the rounding of r leaves alone
+    return unwrap_count(((double)r - peak) * 0.15915494309189533577);
+}
 __global__ void __launch_bounds__(256) k_block_apply(float* __restrict__ pha, const float* __restrict__ dres,
                                                      const double* __restrict__ bin, const int* __restrict__ flist,
                                                      const int* __restrict__ blk, int Nout, int xs) {
     const int f = flist[blockIdx.y];
-    const int jj = blockIdx.x;
+    const int j0 = blockIdx.x * APPLY_ROWS;
+    const int nr = min(APPLY_ROWS, Nout - j0);
     // every pixel lies in a block ((long)(1.0 i / Nout xs) < xs); no blocks: peak 0
-    const double* brow = bin + ((size_t)f * xs + (xs > 0 ? blk[jj] : 0)) * xs;
-    float* prow = pha + ((size_t)f * Nout + jj) * Nout;
-    const float* rrow = dres + ((size_t)f * Nout + jj) * Nout;
-    // r is a float-rounded remainder, so (r - peak) / (2 pi) as a product with
-    // 1 / (2 pi) changes no decision that the rounding of r leaves alone
-    const double inv2pi = 0.15915494309189533577;
+    auto brow = [&](int jj) { return bin + ((size_t)f * xs + (xs > 0 ? blk[jj] : 0)) * xs; };
     if ((Nout & 3) == 0 && (reinterpret_cast<uintptr_t>(pha) & 15) == 0) {  // 16-byte rows
         for (int i4 = threadIdx.x; i4 < Nout / 4; i4 += 256) {
             const int4 b = reinterpret_cast<const int4*>(blk)[i4];
-            const float4 r = reinterpret_cast<const float4*>(rrow)[i4];
-            const double pk0 = xs > 0 ? brow[b.x] : 0.0, pk1 = xs > 0 ? brow[b.y] : 0.0;
-            const double pk2 = xs > 0 ? brow[b.z] : 0.0, pk3 = xs > 0 ? brow[b.w] : 0.0;
-            const double n0 = unwrap_count(((double)r.x - pk0) * inv2pi);
-            const double n1 = unwrap_count(((double)r.y - pk1) * inv2pi);
-            const double n2 = unwrap_count(((double)r.z - pk2) * inv2pi);
-            const double n3 = unwrap_count(((double)r.w - pk3) * inv2pi);
-            if (n0 != 0.0 || n1 != 0.0 || n2 != 0.0 || n3 != 0.0) {
-                float4 p = reinterpret_cast<float4*>(prow)[i4];
-                p.x = (float)fma(2.0 * M_PI, n0, (double)p.x);
-                p.y = (float)fma(2.0 * M_PI, n1, (double)p.y);
-                p.z = (float)fma(2.0 * M_PI, n2, (double)p.z);
-                p.w = (float)fma(2.0 * M_PI, n3, (double)p.w);
-                reinterpret_cast<float4*>(prow)[i4] = p;
+            float4 r[APPLY_ROWS];
+#pragma unroll
+            for (int q = 0; q < APPLY_ROWS; q++)
+                if (q < nr) r[q] = reinterpret_cast<const float4*>(dres + ((size_t)f * Nout + j0 + q) * Nout)[i4];
+#pragma unroll
+            for (int q = 0; q < APPLY_ROWS; q++) {
+                if (q >= nr) continue;
+                const double* br = brow(j0 + q);
+                const double n0 = apply_count(r[q].x, xs > 0 ? br[b.x] : 0.0);
+                const double n1 = apply_count(r[q].y, xs > 0 ? br[b.y] : 0.0);
+                const double n2 = apply_count(r[q].z, xs > 0 ? br[b.z] : 0.0);
+                const double n3 = apply_count(r[q].w, xs > 0 ? br[b.w] : 0.0);
+                if (n0 != 0.0 || n1 != 0.0 || n2 != 0.0 || n3 != 0.0) {
+                    float4* pp = reinterpret_cast<float4*>(pha + ((size_t)f * Nout + j0 + q) * Nout) + i4;
+                    float4 p = *pp;
+                    p.x = (float)fma(2.0 * M_PI, n0, (double)p.x);
+                    p.y = (float)fma(2.0 * M_PI, n1, (double)p.y);
+                    p.z = (float)fma(2.0 * M_PI, n2, (double)p.z);
+                    p.w = (float)fma(2.0 * M_PI, n3, (double)p.w);
+                    *pp = p;
+                }
             }
         }
         return;
     }
-    for (int ii = threadIdx.x; ii < Nout; ii += 256) {
-        const double peak = xs > 0 ? brow[blk[ii]] : 0.0;
-        const double n = unwrap_count(((double)rrow[ii] - peak) * inv2pi);
-        if (n != 0.0) prow[ii] = (float)fma(2.0 * M_PI, n, (double)prow[ii]);
+    for (int q = 0; q < nr; q++) {
+        const double* br = brow(j0 + q);
+        float* prow = pha + ((size_t)f * Nout + j0 + q) * Nout;
+        const float* rrow = dres + ((size_t)f * Nout + j0 + q) * Nout;
+        for (int ii = threadIdx.x; ii < Nout; ii += 256) {
+            const double n = apply_count(rrow[ii], xs > 0 ? br[blk[ii]] : 0.0);
+            if (n != 0.0) prow[ii] = (float)fma(2.0 * M_PI, n, (double)prow[ii]);
+        }
     }
 }
 
 hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,
                               const int* blk, int nfields, int Nout, int xs, hipStream_t st) {
     if (nfields <= 0 || Nout <= 0) return hipSuccess;
-    dim3 grid(Nout, nfields);
+    dim3 grid((Nout + APPLY_ROWS - 1) / APPLY_ROWS, nfields);
     k_block_apply<<<grid, 256, 0, st>>>(pha, dres, bin, field_list, blk, Nout, xs);
     return hipGetLastError();
 }
