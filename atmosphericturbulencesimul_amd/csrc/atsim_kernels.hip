@@ -1595,8 +1595,14 @@ __global__ void __launch_bounds__(256) k_block_counts(const double* __restrict__
         reinterpret_cast<float*>(ch)[(size_t)f * nb + b] = c;
         a = fabsf(c);
     }
+    // the workgroup's largest |count|, then one atomic per workgroup (atomics
+    // on one field's word serialise in L2)
     for (int o = 32; o > 0; o >>= 1) a = fmaxf(a, __shfl_down(a, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&cmax[blockIdx.y], __float_as_uint(a));
+    __shared__ float wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(&cmax[blockIdx.y], __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 // The iterative fix-up from the first round's peaks, counts and largest
