@@ -482,9 +482,9 @@ k_col_screen_split(float2* __restrict__ data, const float2* __restrict__ tfp, co
     // every row) of one field, fields next, and each XCD takes a contiguous
     // eighth of them: the tiles that share 128-byte lines, and every field's
     // phase-plane tiles, are read through one XCD's L2 at about the same
-    // time (2048^2 x 8 fields: 262 us and 1.19 x the algorithmic bytes,
-    // against 266 us and 1.27 x with single tiles and 285 us with 16-index
-    // chunks)
+    // time (2048^2 x 8 fields: 262-265 us and 1.20-1.22 x the algorithmic
+    // bytes, against 266 us and 1.27 x with single tiles and 285 us with
+    // 16-index chunks)
     constexpr int TG = 4;
     static_assert((N / C) % TG == 0, "column tiles come in groups of 4");
     const int lin = xcd_slab(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
