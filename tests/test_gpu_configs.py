@@ -105,6 +105,40 @@ def test_4096_one_pass_per_gap():
         cx.close()
 
 
+def test_8192_one_gap_against_a_torch_fp64_step():
+    # at 8192^2 (the row pass with its three table-twiddle stages and the
+    # 2-column column pass): a one-gap frame -- the top layer's screen and the
+    # x factor in the first row pass, the y factor in the column pass, the
+    # ground layer's screen in the extraction -- against the same step in
+    # torch FP64 (numpy's FFT conventions) from the frame's own sampled
+    # screens, at the FP32 field bar 1e-5 (measured 7e-7; the screens come
+    # back rounded to float); a wrong twiddle, transfer factor or layer order
+    # is an O(1) error.
+    from atmosphericturbulencesimul_amd.api import Context
+    print("case: 8192^2, M=8192, 2 layers (1 gap), 0.5 um, 1 frame", flush=True)
+    N, M, dx, lam, z = 8192, 8192, 0.01, 0.5e-6, 5000.0
+    cx = Context(0)
+    sim, _ = _gpu_masters_case(cx, N, M, [0.0, z], [0.6, 0.4], [10.0, 20.0], [0.3, 1.9], [lam], lam, dx, 0.12,
+                               20.0, 0.01, seed=0xA75 + 5400)
+    assert sim.gaps() == 1
+    field = torch.empty((1, 1, N, N), dtype=torch.complex64, device="cuda")
+    sim.frame(field)
+    scr = torch.empty((1, 2, N * N), dtype=torch.float32, device="cuda")
+    cx._chk(cx.L.atsim_sim_layer_screens(sim.h, scr.data_ptr()))
+    cx.sync()
+    v = scr.double().view(2, N, N)  # layer 0 at the ground, layer 1 at z
+    kt = torch.fft.fftfreq(N, d=1.0 / N, device="cuda", dtype=torch.float64)  # signed frequency indices
+    coeff = math.pi * z * lam / (dx * N) ** 2
+    H = torch.exp(-1j * coeff * (kt[:, None] ** 2 + kt[None, :] ** 2))
+    E = torch.fft.ifft2(torch.fft.fft2(torch.exp(1j * v[1])) * H) * torch.exp(1j * v[0])
+    del H
+    rel = float(torch.linalg.vector_norm(field[0, 0].to(torch.complex128) - E) / torch.linalg.vector_norm(E))
+    print(f"field rel L2 vs torch FP64 {rel:.2e}", flush=True)
+    assert rel < 1e-5
+    sim.close()
+    cx.close()
+
+
 def test_1024_seven_layers_five_wavelengths(ctx, oracle):
     print("case: 1024^2, M=4096, 7 layers, 5 wavelengths, 2 frames", flush=True)
     N, M, dx = 1024, 4096, 0.01
