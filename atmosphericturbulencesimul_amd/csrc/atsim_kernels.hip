@@ -1293,60 +1293,89 @@ __device__ __forceinline__ float wave_inclusive_sum(float x) {
 }
 
 // ------------------------------------------------------------------ extraction
-// One workgroup per (256-pixel row segment, row, direction x wavelength
-// chunk of <= 8), one pixel per thread.  For every layer the 4 master rows
-// under the segment are staged in LDS and the pixel's sample (FP64) is shared
-// by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l and the
-// screen of the last layer group, applied here after the last Fresnel gap.
-constexpr int EXT_T = 256;
-constexpr int EXT_LC = 5;   // layers staged per LDS fill (33 KB of LDS: 4 workgroups per CU)
+// One workgroup per tile of 4 rows x 64 pixels (one wave per row, one pixel
+// per lane) and direction x wavelength chunk of <= 8.  For every layer the 7
+// master rows under the tile are staged in LDS, each x-filtered value (FP64)
+// is made once and shared by the 4 rows that read it, and the pixel's sample
+// is shared by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l
+// and the screen of the last layer group, applied here after the last
+// Fresnel gap.
+constexpr int EXT_W = 64;              // pixels per tile row (one wave)
+constexpr int EXT_R = 4;               // rows per tile
+constexpr int EXT_T = EXT_W * EXT_R;   // threads
+constexpr int EXT_SR = EXT_R + 3;      // staged master rows
+constexpr int EXT_SW = EXT_W + 4;      // staged row stride (floats)
+constexpr int EXT_LC = 5;              // layers staged per LDS fill
+constexpr int EXT_BC = 8;              // block columns a tile row can touch (blocks are >= 20 pixels wide)
 template <bool FULL>
 __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
-    __shared__ float S[EXT_LC][4][EXT_T + 4];
-    __shared__ double bs[8][2][16];
-    __shared__ LayerSample sls[MAX_LAYERS];  // this direction's layer samples
-    __shared__ double ss[MAX_LAYERS][8];     // phase scales of this chunk's wavelengths (0 past Wc)
+    __shared__ float S[EXT_LC][EXT_SR][EXT_SW];   // staged master rows
+    __shared__ double H[EXT_LC][EXT_SR][EXT_W];   // their x-filtered values
+    __shared__ double bs[2][8][2][EXT_BC];        // [block row - jb0][w][cos, sin][block column - b0]
+    __shared__ LayerSample sls[MAX_LAYERS];       // this direction's layer samples
+    __shared__ double ss[MAX_LAYERS][8];          // phase scales of this chunk's wavelengths (0 past Wc)
+    __shared__ int ysq[MAX_LAYERS], xsq[MAX_LAYERS];  // wrapped first staged row / column per layer
+    __shared__ int um[8];                         // unwrap rule of each wavelength of the chunk
     const int nchunks = (a.W + 7) / 8;
-    const int seg = blockIdx.x, row = blockIdx.y;
-    const int d = blockIdx.z / nchunks, chunk = blockIdx.z - d * nchunks;
+    const int d = blockIdx.y / nchunks, chunk = blockIdx.y - d * nchunks;
     const int w_begin = chunk * 8;
     const int Wc = min(8, a.W - w_begin);
+    const int nrows = FULL ? a.N : a.Nout, ncols = nrows;
+    // tiles: with a multiple of 8 tile columns each XCD takes a stripe of
+    // adjacent tile columns and walks it down the rows, so the master lines
+    // that neighbouring tiles share (3 rows below, 3 columns right) are read
+    // through one L2 close together in time
+    const int ntx = (ncols + EXT_W - 1) / EXT_W;
+    int tx, ty;
+    if ((ntx & 7) == 0) {
+        const int sw = ntx >> 3, idx = blockIdx.x >> 3;
+        ty = idx / sw;
+        tx = (blockIdx.x & 7) * sw + (idx - ty * sw);
+    } else {
+        ty = blockIdx.x / ntx;
+        tx = blockIdx.x - ty * ntx;
+    }
+    // wave index (uniform: the tile row and the staging loops stay scalar)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int o = (a.N - a.Nout) / 2;
-    const int jf = FULL ? row : row + o;
+    const int r0 = ty * EXT_R, row = r0 + wv;
+    const int c0 = tx * EXT_W;
+    const bool valid = row < nrows && c0 + lane < ncols;
+    const int jf0 = FULL ? r0 : r0 + o;  // full-grid row of the tile's first row
+    const int jf = jf0 + wv;
     const int jj = jf - o;
-    const bool rowin = jj >= 0 && jj < a.Nout;
-    const int ncols = FULL ? a.N : a.Nout;
-    const int c0 = seg * EXT_T;
-    const int cc = c0 + threadIdx.x;
-    const bool valid = cc < ncols;
-    const int xfirst = FULL ? c0 : c0 + o;  // full-grid x of the segment's first pixel
-    const int ifl = xfirst + threadIdx.x;
+    const bool rowin = row < nrows && jj >= 0 && jj < a.Nout;
+    const int xfirst = FULL ? c0 : c0 + o;  // full-grid x of the tile's first pixel
+    const int ifl = xfirst + lane;
     const int ii = ifl - o;
     const bool colin = valid && ii >= 0 && ii < a.Nout;
     const int xs = a.xs;
     const int jj2 = rowin ? (int)((double)jj / a.Nout * xs) : -1;
-    const bool blockrow = rowin && jj2 >= 0 && jj2 < xs && a.amplitude && a.bsum != nullptr;
-    const int ii_lo = max(xfirst - o, 0);
-    const int b0 = (xs > 0 && ii_lo < a.Nout) ? (int)((double)ii_lo / a.Nout * xs) : 0;
+    // block row of the tile's first row in the output (the 4 rows span at
+    // most 2 block rows) and block column of its first pixel in the output
+    const int jjs = max(jf0 - o, 0), iis = max(xfirst - o, 0);
+    const int jb0 = (xs > 0 && jjs < a.Nout) ? (int)((double)jjs / a.Nout * xs) : 0;
+    const int b0 = (xs > 0 && iis < a.Nout) ? (int)((double)iis / a.Nout * xs) : 0;
+    const int jb = jj2 - jb0;
+    const bool blockrow = rowin && jj2 >= 0 && jj2 < xs && jb >= 0 && jb < 2 && a.amplitude && a.bsum != nullptr;
     // block column of this pixel (AT.c:3611: (long)(1.0*ii/Nout*xs)); -1 if it adds to no block
     const int ii2 = (blockrow && colin) ? a.blk[ii] : -1;
     const int L = a.all.L;
     const int M = (int)a.all.M;
     // per-frame layer parameters, read once into LDS (global loads of uniform
     // values would otherwise be issued per lane inside the layer loop)
-    __shared__ int ysq[MAX_LAYERS], xsq[MAX_LAYERS];  // wrapped first staged row / column per layer
     for (int k = threadIdx.x; k < L; k += EXT_T) {
         const LayerSample p = a.all.ls[d * L + k];
         sls[k] = p;
-        ysq[k] = mod_m(p.y0 + jf - 1, M);
+        ysq[k] = mod_m(p.y0 + jf0 - 1, M);
         xsq[k] = mod_m(p.x0 + xfirst - 1, M);
     }
     for (int k = threadIdx.x; k < 8 * L; k += EXT_T) {
         const int w = k / L, l = k - w * L;
         ss[l][w] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
     }
-    __shared__ int um[8];  // unwrap rule of each wavelength of the chunk
     if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
+    for (int k = threadIdx.x; k < 2 * 8 * 2 * EXT_BC; k += EXT_T) (&bs[0][0][0][0])[k] = 0.0;
     __syncthreads();
     // the fields are needed only after the layers: issue their loads first
     float2 Ein[8];
@@ -1354,71 +1383,77 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     for (int w = 0; w < 8; w++)
         Ein[w] = (valid && w < Wc && a.fields) ? a.fields[((size_t)(d * a.W + w_begin + w) * a.N + jf) * a.N + ifl]
                                                : make_float2(1.f, 0.f);
-    for (int k = threadIdx.x; k < 8 * 2 * 16; k += EXT_T) (&bs[0][0][0])[k] = 0.0;
     double PHI[8], PL[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
     for (int l0 = 0; l0 < L; l0 += EXT_LC) {
         const int nl = min(EXT_LC, L - l0);
+        __syncthreads();  // the previous fill's values are read
+        // lane c of wave wv stages column c of the 7 rows of layers wv, wv + 4
+        for (int q = wv; q < nl; q += EXT_R) {
+            const int ys = __builtin_amdgcn_readfirstlane(ysq[l0 + q]);
+            const int x = wrap_col(__builtin_amdgcn_readfirstlane(xsq[l0 + q]) + lane, M);
+            const float* m = a.all.masters + (size_t)(l0 + q) * M * M;
+            float t[EXT_SR];
+#pragma unroll
+            for (int r = 0; r < EXT_SR; r++) t[r] = m[(size_t)wrap_col(ys + r, M) * M + x];
+#pragma unroll
+            for (int r = 0; r < EXT_SR; r++) S[q][r][lane] = t[r];
+        }
+        // the 3 columns right of the tile, one element per thread
+        for (int e = threadIdx.x; e < nl * EXT_SR * 3; e += EXT_T) {
+            const int qr = e / 3, c = EXT_W + (e - qr * 3);
+            const int q = qr / EXT_SR, r = qr - q * EXT_SR;
+            S[q][r][c] = a.all.masters[(size_t)(l0 + q) * M * M + (size_t)wrap_col(ysq[l0 + q] + r, M) * M +
+                                       wrap_col(xsq[l0 + q] + c, M)];
+        }
         __syncthreads();
-        {
-            // main columns: lane c stages column c of the 4 rows of EXT_G layers at a time
-            constexpr int EXT_G = 5;
-            for (int q0 = 0; q0 < nl; q0 += EXT_G) {
-                float t[EXT_G][4];
-#pragma unroll
-                for (int g = 0; g < EXT_G; g++) {
-                    const int q = q0 + g;
-                    if (q < nl) {
-                        const int ys = __builtin_amdgcn_readfirstlane(ysq[l0 + q]);
-                        const int xs0 = __builtin_amdgcn_readfirstlane(xsq[l0 + q]);
-                        load_col4(t[g], a.all.masters + (size_t)(l0 + q) * M * M, M, ys,
-                                  wrap_col(xs0 + (int)threadIdx.x, M));
-                    }
-                }
-#pragma unroll
-                for (int g = 0; g < EXT_G; g++)
-                    if (q0 + g < nl) {
-#pragma unroll
-                        for (int b = 0; b < 4; b++) S[q0 + g][b][threadIdx.x] = t[g][b];
-                    }
-            }
-            stage_tail(&S[0][0][0], EXT_T + 4, a.all.masters + (size_t)l0 * M * M, (size_t)M * M, M, &ysq[l0],
-                       &xsq[l0], nl, EXT_T, 3, threadIdx.x, EXT_T);
+        // x-filtered values: wave wv makes staged (layer, row) pairs wv, wv + 4, ...
+        for (int k = wv; k < nl * EXT_SR; k += EXT_R) {
+            const int q = k / EXT_SR, r = k - q * EXT_SR;  // uniform
+            const LayerSample& p = sls[l0 + q];
+            const float* s = &S[q][r][lane];
+            double h = p.wx[0] * (double)s[0];
+            h = fma(p.wx[1], (double)s[1], h);
+            h = fma(p.wx[2], (double)s[2], h);
+            H[q][r][lane] = fma(p.wx[3], (double)s[3], h);
         }
         __syncthreads();
         if (valid) {
-            for (int q = 0; q < nl; q++) {
-                const int l = l0 + q;
-                const LayerSample& p = sls[l];
+            // the pixel's FP64 sample of staged layer q: its 4 x-filtered rows
+            auto sample = [&](int q) {
+                const LayerSample& p = sls[l0 + q];
                 double v = 0.0;
 #pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const float* r = &S[q][b][threadIdx.x];
-                    double h = p.wx[0] * (double)r[0];
-                    h = fma(p.wx[1], (double)r[1], h);
-                    h = fma(p.wx[2], (double)r[2], h);
-                    h = fma(p.wx[3], (double)r[3], h);
-                    v = fma(p.wy[b], h, v);
-                }
-                // wavelengths past Wc have zero scales: no per-wavelength tests
-                const double* sl = ss[l];
-                if (l >= a.last_first && l < a.last_first + a.last_count) {  // uniform
+                for (int b = 0; b < 4; b++) v = fma(p.wy[b], H[q][wv + b][lane], v);
+                return v;
+            };
+            // staged layers lo .. hi - 1 belong to the last group (uniform
+            // bounds).  Two loops with straight bodies: one branch per layer
+            // would make the compiler copy the accumulators it does not
+            // update.  Wavelengths past Wc have zero scales: no per-wavelength
+            // tests.
+            const int lo = min(max(a.last_first - l0, 0), nl);
+            const int hi = min(max(a.last_first + a.last_count - l0, lo), nl);
+            for (int k = 0; k < nl - (hi - lo); k++) {
+                const int q = k < lo ? k : k + (hi - lo);
+                const double v = sample(q);
+                const double* sl = ss[l0 + q];
 #pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        const double sv = sl[w] * v;
-                        PHI[w] += sv;
-                        PL[w] += sv;
-                    }
-                } else {
+                for (int w = 0; w < 8; w++) PHI[w] = fma(sl[w], v, PHI[w]);
+            }
+            for (int q = lo; q < hi; q++) {
+                const double v = sample(q);
+                const double* sl = ss[l0 + q];
 #pragma unroll
-                    for (int w = 0; w < 8; w++) PHI[w] += sl[w] * v;
-                }
+                for (int w = 0; w < 8; w++) PL[w] = fma(sl[w], v, PL[w]);
             }
         }
     }
+    // the total geometric phase: the other layers' sum plus the last group's
+#pragma unroll
+    for (int w = 0; w < 8; w++) PHI[w] += PL[w];
     const double inv2pi = 0.15915494309189533577;
-    const int lane = threadIdx.x & 63;
     // block column of lane + 1 (-1: none, or the last lane) and whether this
     // lane ends a run of one block column
     int next_key = __shfl_down(ii2, 1, 64);
@@ -1488,42 +1523,45 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                 }
             }
         }
-        // per-block sums over the wave: with P the wave's inclusive prefix sum,
-        // a run of lanes s..e of one block sums to P[e] - P[s-1].  Block
-        // columns are non-decreasing along the valid lanes and lanes outside
-        // every block add zero, so the last lane e of each run adds P[e] to its
-        // block and subtracts it from the block of lane e + 1.
+        // per-block sums over the wave (one tile row): with P the wave's
+        // inclusive prefix sum, a run of lanes s..e of one block sums to
+        // P[e] - P[s-1].  Block columns are non-decreasing along the valid
+        // lanes and lanes outside every block add zero, so the last lane e of
+        // each run adds P[e] to its block and subtracts it from the block of
+        // lane e + 1.
         if (blockrow && block_rule) {
             const double px = (double)wave_inclusive_sum(cx), py = (double)wave_inclusive_sum(cy);
             if (run_end) {
-                if (ii2 - b0 < 16) {
-                    atomicAdd(&bs[w][0][ii2 - b0], px);
-                    atomicAdd(&bs[w][1][ii2 - b0], py);
+                if (ii2 - b0 < EXT_BC) {
+                    atomicAdd(&bs[jb][w][0][ii2 - b0], px);
+                    atomicAdd(&bs[jb][w][1][ii2 - b0], py);
                 }
-                if (next_key >= 0 && next_key - b0 < 16) {
-                    atomicAdd(&bs[w][0][next_key - b0], -px);
-                    atomicAdd(&bs[w][1][next_key - b0], -py);
+                if (next_key >= 0 && next_key - b0 < EXT_BC) {
+                    atomicAdd(&bs[jb][w][0][next_key - b0], -px);
+                    atomicAdd(&bs[jb][w][1][next_key - b0], -py);
                 }
             }
         }
     }
     __syncthreads();
-    if (!blockrow) return;
+    if (xs <= 0 || !a.amplitude || a.bsum == nullptr) return;
     const int nb = xs * xs;
-    for (int k = threadIdx.x; k < Wc * 16; k += EXT_T) {
-        const int w = k >> 4, lb = k & 15;
-        if (um[w] == 0 || b0 + lb >= xs) continue;
-        if (bs[w][0][lb] == 0.0 && bs[w][1][lb] == 0.0) continue;
+    for (int k = threadIdx.x; k < 2 * 8 * EXT_BC; k += EXT_T) {
+        const int j = k / (8 * EXT_BC), w = (k / EXT_BC) & 7, lb = k % EXT_BC;
+        if (w >= Wc || um[w] == 0 || jb0 + j >= xs || b0 + lb >= xs) continue;
+        if (bs[j][w][0][lb] == 0.0 && bs[j][w][1][lb] == 0.0) continue;
         const int f = d * a.W + w_begin + w;
-        double* dst = a.bsum + ((size_t)f * nb + (size_t)jj2 * xs + b0 + lb) * 2;
-        atomicAdd(dst, bs[w][0][lb]);
-        atomicAdd(dst + 1, bs[w][1][lb]);
+        double* dst = a.bsum + ((size_t)f * nb + (size_t)(jb0 + j) * xs + b0 + lb) * 2;
+        atomicAdd(dst, bs[j][w][0][lb]);
+        atomicAdd(dst + 1, bs[j][w][1][lb]);
     }
 }
 
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st) {
-    const int ncols = full ? a.N : a.Nout;
-    dim3 grid((ncols + EXT_T - 1) / EXT_T, full ? a.N : a.Nout, a.D * ((a.W + 7) / 8)), block(EXT_T);
+    const int n = full ? a.N : a.Nout;
+    const long tiles = (long)((n + EXT_W - 1) / EXT_W) * ((n + EXT_R - 1) / EXT_R);
+    if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)tiles, a.D * ((a.W + 7) / 8)), block(EXT_T);
     if (full) k_extract<true><<<grid, block, 0, st>>>(a);
     else k_extract<false><<<grid, block, 0, st>>>(a);
     return hipGetLastError();
