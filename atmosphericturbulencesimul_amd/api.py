@@ -425,7 +425,8 @@ class Simulation:
 
     # row / col: passes with one Fresnel factor; row2 / col2: passes with a
     # factor, a layer group's screen and a factor (one pass per gap)
-    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3, "row2": 4, "col2": 5, "phase": 6, "stats": 7}
+    PROF_KINDS = {"row": 0, "col": 1, "extract": 2, "block": 3, "row2": 4, "col2": 5, "phase": 6, "stats": 7,
+                  "geometry": 8}
 
     def profile(self, enable=True):
         self.ctx._chk(self.ctx.L.atsim_sim_profile(self.h, int(enable)))

@@ -35,6 +35,10 @@ struct atsim_ctx {
     int gap_passes = 1;  // 1: one pass over the field per gap (x / y order alternating), 2: a row and a column pass
     int col2_e = 0;      // elements per thread of the column passes with a screen (0 = default, 16, 32)
     int graph = 0;   // 1: frames replay a captured HIP graph of their kernels
+    // 1: with factored phase multipliers the geometric phase is sampled into
+    // two planes beside the Fresnel passes and the extraction reads them;
+    // 2: the extraction samples the layers itself
+    int extract_planes = 1;
     unsigned tuning_gen = 0;  // bumped by every accepted atsim_set_tuning: captured frame graphs are stale
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
@@ -346,6 +350,9 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "graph must be 0 or 1");
         c->graph = value;
+    } else if (!strcmp(key, "extract_planes")) {
+        if (value < 0 || value > 2) return fail(c, ATSIM_EINVAL, "extract_planes must be 0, 1 or 2");
+        c->extract_planes = value == 0 ? 1 : value;
     } else {
         return fail(c, ATSIM_EINVAL, "unknown tuning key '%s'", key);
     }
@@ -362,6 +369,7 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!strcmp(key, "gap_passes")) return c->gap_passes;
     if (!strcmp(key, "col2_e")) return c->col2_e;
     if (!strcmp(key, "graph")) return c->graph;
+    if (!strcmp(key, "extract_planes")) return c->extract_planes;
     return ATSIM_EINVAL;
 }
 
@@ -832,6 +840,13 @@ struct atsim_sim {
     static constexpr int RING = 8;
     double* s_dev = nullptr;
     int* unwrap_dev = nullptr;
+    // phase multipliers that factor as s_wl = geo_a[w] geo_b[l]: the planes
+    // of the geometric phase's layer sums ([D][2][N][N] FP64), made beside
+    // the Fresnel passes
+    bool factored = false;
+    double* geo_a = nullptr;  // [W]
+    double* geo_b = nullptr;  // [L]
+    double* vgeo = nullptr;
     // extra streams of the multi-stream Fresnel passes (set k > 0 on xs_stream[k - 1])
     static constexpr int MAX_SETS = 4;
     hipStream_t xs_stream[MAX_SETS - 1] = {};
@@ -887,7 +902,8 @@ enum {
     PROF_COL_GAP2 = 5,  // column pass: y factor, screen, y factor
     PROF_PHASE = 6,     // sampled phase planes of the column passes' layers
     PROF_STATS = 7,     // per-frame output statistics
-    PROF_KINDS = 8
+    PROF_GEOMETRY = 8,  // the factored geometric phase's planes
+    PROF_KINDS = 9
 };
 
 static hipEvent_t prof_event(atsim_sim* s) {
@@ -942,6 +958,9 @@ static void sim_free(atsim_sim* s) {
         delete[] s->ls_ev;
     }
     hipFree(s->s_dev);
+    hipFree(s->geo_a);
+    hipFree(s->geo_b);
+    hipFree(s->vgeo);
     hipFree(s->unwrap_dev);
     hipFree(s->dres);
     for (int k = 0; k < atsim_sim::MAX_SETS - 1; k++) {
@@ -1031,6 +1050,22 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     SIMALLOC(s->s_dev, sizeof(double) * s->W * s->L);
     SIMALLOC(s->unwrap_dev, sizeof(int) * s->W);
     SIMALLOC(s->blk, sizeof(int) * (s->Nout > 0 ? s->Nout : 1));
+    // multipliers that factor exactly as s_wl = a_w b_l (a_w = s_w0,
+    // b_l = s_0l / s_00; the reference's lambda_ref / lambda rows do, with
+    // b_l = 1): the extraction's geometric phase then needs two planes
+    std::vector<double> geo_a(s->W), geo_b(s->L);
+    s->factored = s->W > 0 && s->L > 0 && s->scoeff[0] != 0.0;
+    for (long l = 0; s->factored && l < s->L; l++) geo_b[l] = s->scoeff[l] / s->scoeff[0];
+    for (long w = 0; s->factored && w < s->W; w++) {
+        geo_a[w] = s->scoeff[w * s->L];
+        for (long l = 0; l < s->L; l++)
+            if (geo_a[w] * geo_b[l] != s->scoeff[w * s->L + l]) s->factored = false;
+    }
+    if (s->factored) {
+        SIMALLOC(s->geo_a, sizeof(double) * s->W);
+        SIMALLOC(s->geo_b, sizeof(double) * s->L);
+        SIMALLOC(s->vgeo, sizeof(double) * 2 * s->D * nn);
+    }
     if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
     if (r == ATSIM_OK && (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1090,6 +1125,8 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     for (long i = 0; i < s->Nout; i++) blk[i] = (int)((double)i / s->Nout * s->xs);
     if (r == ATSIM_OK) {
         if (copy_on_stream(c, s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size()) != hipSuccess ||
+            (s->factored && (copy_on_stream(c, s->geo_a, geo_a.data(), sizeof(double) * s->W) != hipSuccess ||
+                             copy_on_stream(c, s->geo_b, geo_b.data(), sizeof(double) * s->L) != hipSuccess)) ||
             copy_on_stream(c, s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W) != hipSuccess ||
             copy_on_stream(c, s->blk, blk.data(), sizeof(int) * blk.size()) != hipSuccess ||
             (s->nblockfields > 0 &&
@@ -1264,6 +1301,43 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     const int nsets = (int)std::min<long>(c->streams, W);
     const bool fused = use_gap_fusion(s);
     const long nsteps = G <= 0 ? 0 : (fused ? G + 1 : 2 * G);
+    ExtractArgs a{};
+    a.fields = s->propagate ? s->fields : nullptr;
+    a.field_out = (float2*)d_field;
+    a.pha = d_pha;
+    a.amp = d_amp;
+    a.dres = s->dres;
+    a.blk = s->blk;
+    a.bsum = s->bsum;
+    a.unwrap_mode = s->unwrap_dev;
+    a.all = sa;
+    a.all.wlo = a.all.wcnt = 0;
+    a.all.lfirst = 0;
+    a.all.lcount = (int)L;
+    if (s->propagate) {
+        a.last_first = s->groups[G].lfirst;
+        a.last_count = s->groups[G].lcount;
+    } else {
+        a.last_first = 0;
+        a.last_count = (int)L;
+    }
+    a.N = (int)s->N;
+    a.Nout = (int)s->Nout;
+    a.W = (int)W;
+    a.D = (int)D;
+    a.amplitude = s->amplitude;
+    a.xs = s->xs;
+    // factored multipliers: the geometric phase's planes, made beside the
+    // passes on the second set's stream (or before the extraction on one)
+    const bool planes = s->factored && c->extract_planes == 1;
+    const bool full = d_field != nullptr;
+    if (planes) {
+        a.geo_a = s->geo_a;
+        a.geo_b = s->geo_b;
+        a.vr = s->vgeo;
+        a.vl = s->vgeo + (size_t)D * s->N * s->N;
+    }
+    bool geometry_done = false;
     auto phase = [&](hipStream_t st) {
         return launch_layer_phase(s->vq, s->masters, s->ls_dev, s->slot_layer_dev, s->nslots, s->N, M, L, D, st);
     };
@@ -1274,12 +1348,18 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         // runs beside another's column pass (memory-bound).  All join the
         // context stream before extraction.  The phase planes of the column
         // passes are sampled on set 1's stream beside set 0's first (row)
-        // pass; every other set's first column pass waits for them.
+        // pass; every other set's first column pass waits for them.  The
+        // extraction's geometric-phase planes follow on that stream.
         HIPCHK(c, hipEventRecord(s->ev_fork, ms));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
         if (fused) {
             HIPCHK(c, phase(s->xs_stream[0]));
             HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
+        }
+        if (planes) {
+            // joins the context stream with set 1's chain
+            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0]));
+            geometry_done = true;
         }
         for (long step = 0; step < nsteps; step++) {
             for (int k = 0; k < nsets; k++) {
@@ -1311,34 +1391,14 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
             prof_end(s, pr);
         }
     }
-    ExtractArgs a{};
-    a.fields = s->propagate ? s->fields : nullptr;
-    a.field_out = (float2*)d_field;
-    a.pha = d_pha;
-    a.amp = d_amp;
-    a.dres = s->dres;
-    a.blk = s->blk;
-    a.bsum = s->bsum;
-    a.unwrap_mode = s->unwrap_dev;
-    a.all = sa;
-    a.all.lfirst = 0;
-    a.all.lcount = (int)L;
-    if (s->propagate) {
-        a.last_first = s->groups[G].lfirst;
-        a.last_count = s->groups[G].lcount;
-    } else {
-        a.last_first = 0;
-        a.last_count = (int)L;
+    if (planes && !geometry_done) {
+        long pg = prof_begin(s, PROF_GEOMETRY);
+        HIPCHK(c, launch_extract_geometry(a, full, ms));
+        prof_end(s, pg);
     }
-    a.N = (int)s->N;
-    a.Nout = (int)s->Nout;
-    a.W = (int)W;
-    a.D = (int)D;
-    a.amplitude = s->amplitude;
-    a.xs = s->xs;
     if (s->nblockfields > 0 && s->xs > 0) HIPCHK(c, launch_zero_f64(s->bsum, F * s->xs * s->xs * 2, ms));
     long pe = prof_begin(s, PROF_EXTRACT);
-    HIPCHK(c, launch_extract(a, d_field != nullptr, ms));
+    HIPCHK(c, launch_extract(a, full, ms));
     prof_end(s, pe);
     if (s->nblockfields > 0) {
         long pb = prof_begin(s, PROF_BLOCK);

@@ -1300,6 +1300,12 @@ __device__ __forceinline__ float wave_inclusive_sum(float x) {
 // is shared by the chunk's wavelengths: geometric phase PHI_w = sum_l s_wl v_l
 // and the screen of the last layer group, applied here after the last
 // Fresnel gap.
+//
+// MODE 0 does all of that.  When the phase multipliers factor as
+// s_wl = a_w b_l, the sums need the layers only through two planes per
+// direction, Vr = sum b_l v_l over the layers before the last group and Vl
+// over the last group: k_geo_planes writes them, and MODE 2 reads them
+// instead of sampling (PHI_w = a_w Vr + a_w Vl).
 constexpr int EXT_W = 64;              // pixels per tile row (one wave)
 constexpr int EXT_R = 4;               // rows per tile
 constexpr int EXT_T = EXT_W * EXT_R;   // threads
@@ -1307,8 +1313,11 @@ constexpr int EXT_SR = EXT_R + 3;      // staged master rows
 constexpr int EXT_SW = EXT_W + 4;      // staged row stride (floats)
 constexpr int EXT_LC = 5;              // layers staged per LDS fill
 constexpr int EXT_BC = 8;              // block columns a tile row can touch (blocks are >= 20 pixels wide)
-template <bool FULL>
+template <bool FULL, int MODE>
 __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
+    static_assert(MODE == 0 || MODE == 2, "sampling or planes");
+    constexpr bool SAMPLE = MODE == 0, OUTPUT = true;
+    constexpr int NW = 8;  // accumulated wavelengths
     __shared__ float S[EXT_LC][EXT_SR][EXT_SW];   // staged master rows
     __shared__ double H[EXT_LC][EXT_SR][EXT_W];   // their x-filtered values
     __shared__ double bs[2][8][2][EXT_BC];        // [block row - jb0][w][cos, sin][block column - b0]
@@ -1364,29 +1373,45 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     const int M = (int)a.all.M;
     // per-frame layer parameters, read once into LDS (global loads of uniform
     // values would otherwise be issued per lane inside the layer loop)
-    for (int k = threadIdx.x; k < L; k += EXT_T) {
-        const LayerSample p = a.all.ls[d * L + k];
-        sls[k] = p;
-        ysq[k] = mod_m(p.y0 + jf0 - 1, M);
-        xsq[k] = mod_m(p.x0 + xfirst - 1, M);
+    if (SAMPLE) {
+        for (int k = threadIdx.x; k < L; k += EXT_T) {
+            const LayerSample p = a.all.ls[d * L + k];
+            sls[k] = p;
+            ysq[k] = mod_m(p.y0 + jf0 - 1, M);
+            xsq[k] = mod_m(p.x0 + xfirst - 1, M);
+        }
+        for (int k = threadIdx.x; k < NW * L; k += EXT_T) {
+            const int w = k / L, l = k - w * L;
+            ss[l][w] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
+        }
     }
-    for (int k = threadIdx.x; k < 8 * L; k += EXT_T) {
-        const int w = k / L, l = k - w * L;
-        ss[l][w] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
+    if (OUTPUT) {
+        if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
+        for (int k = threadIdx.x; k < 2 * 8 * 2 * EXT_BC; k += EXT_T) (&bs[0][0][0][0])[k] = 0.0;
     }
-    if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
-    for (int k = threadIdx.x; k < 2 * 8 * 2 * EXT_BC; k += EXT_T) (&bs[0][0][0][0])[k] = 0.0;
     __syncthreads();
     // the fields are needed only after the layers: issue their loads first
     float2 Ein[8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
-        Ein[w] = (valid && w < Wc && a.fields) ? a.fields[((size_t)(d * a.W + w_begin + w) * a.N + jf) * a.N + ifl]
-                                               : make_float2(1.f, 0.f);
+        Ein[w] = (OUTPUT && valid && w < Wc && a.fields)
+                     ? a.fields[((size_t)(d * a.W + w_begin + w) * a.N + jf) * a.N + ifl]
+                     : make_float2(1.f, 0.f);
     double PHI[8], PL[8];
 #pragma unroll
     for (int w = 0; w < 8; w++) { PHI[w] = 0.0; PL[w] = 0.0; }
-    for (int l0 = 0; l0 < L; l0 += EXT_LC) {
+    // planes of the factored sums: [D][N][N] at full-grid positions
+    const size_t gi = ((size_t)d * a.N + jf) * a.N + ifl;
+    if (MODE == 2 && valid) {
+        const double vr = a.vr[gi], vl = a.vl[gi];
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            const double aw = w < Wc ? a.geo_a[w_begin + w] : 0.0;
+            PHI[w] = aw * vr;
+            PL[w] = aw * vl;
+        }
+    }
+    for (int l0 = 0; SAMPLE && l0 < L; l0 += EXT_LC) {
         const int nl = min(EXT_LC, L - l0);
         __syncthreads();  // the previous fill's values are read
         // lane c of wave wv stages column c of the 7 rows of layers wv, wv + 4
@@ -1440,13 +1465,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                 const double v = sample(q);
                 const double* sl = ss[l0 + q];
 #pragma unroll
-                for (int w = 0; w < 8; w++) PHI[w] = fma(sl[w], v, PHI[w]);
+                for (int w = 0; w < NW; w++) PHI[w] = fma(sl[w], v, PHI[w]);
             }
             for (int q = lo; q < hi; q++) {
                 const double v = sample(q);
                 const double* sl = ss[l0 + q];
 #pragma unroll
-                for (int w = 0; w < 8; w++) PL[w] = fma(sl[w], v, PL[w]);
+                for (int w = 0; w < NW; w++) PL[w] = fma(sl[w], v, PL[w]);
             }
         }
     }
@@ -1557,13 +1582,113 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     }
 }
 
+// The factored geometric phase's planes for the extraction's MODE 2:
+// vr / vl [D][N][N] (full-grid positions) = sum_l b_l v_l over the layers
+// outside the last group [lf, lf + lc) and over that group, for full-grid
+// rows r_first .. r_first + nrows - 1 and columns c_first .. + ncols - 1.
+// A workgroup makes 4 rows x 256 columns: per layer it stages the 7 master
+// rows under them (LDS, 7 KB) and each thread makes its column's 7
+// x-filtered values once and its 4 samples (the taps and order of the
+// extraction's sampling).  With a multiple of 8 tile columns each XCD walks
+// its stripe of tile columns down the rows (tiles share 3 master rows).
+constexpr int GEO_W = 256, GEO_R = 4, GEO_SR = GEO_R + 3;
+__global__ void __launch_bounds__(GEO_W)
+k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __restrict__ masters,
+             const LayerSample* __restrict__ ls, const double* __restrict__ geo_b, int N, int M, int L, int lf, int lc,
+             int r_first, int nrows, int c_first, int ncols) {
+    __shared__ float S[GEO_SR][GEO_W + 4];
+    const int ntx = (ncols + GEO_W - 1) / GEO_W;
+    int tx, ty;
+    if ((ntx & 7) == 0) {
+        const int sw = ntx >> 3, idx = blockIdx.x >> 3;
+        ty = idx / sw;
+        tx = (blockIdx.x & 7) * sw + (idx - ty * sw);
+    } else {
+        ty = blockIdx.x / ntx;
+        tx = blockIdx.x - ty * ntx;
+    }
+    const int d = blockIdx.y, t = threadIdx.x;
+    const int jf0 = r_first + ty * GEO_R, cf0 = c_first + tx * GEO_W;
+    const bool colin = tx * GEO_W + t < ncols;
+    double acc_r[GEO_R], acc_l[GEO_R];
+#pragma unroll
+    for (int q = 0; q < GEO_R; q++) acc_r[q] = acc_l[q] = 0.0;
+    // one layer: stage, then this column's 4 samples into acc
+    auto layer = [&](int l, double(&acc)[GEO_R]) {
+        const LayerSample& p = ls[d * L + l];
+        const int ys = mod_m(p.y0 + jf0 - 1, M), xs0 = mod_m(p.x0 + cf0 - 1, M);  // uniform
+        const float* m = masters + (size_t)l * M * M;
+        __syncthreads();  // the previous layer's staged rows are read
+        {
+            const int x = wrap_col(xs0 + t, M);
+            float v[GEO_SR];
+#pragma unroll
+            for (int r = 0; r < GEO_SR; r++) v[r] = m[(size_t)wrap_col(ys + r, M) * M + x];
+#pragma unroll
+            for (int r = 0; r < GEO_SR; r++) S[r][t] = v[r];
+            if (t < GEO_SR * 3) {
+                const int r = t / 3, c = GEO_W + (t - r * 3);
+                S[r][c] = m[(size_t)wrap_col(ys + r, M) * M + wrap_col(xs0 + c, M)];
+            }
+        }
+        __syncthreads();
+        double h[GEO_SR];
+#pragma unroll
+        for (int r = 0; r < GEO_SR; r++) {
+            const float* q = &S[r][t];
+            double x = p.wx[0] * (double)q[0];
+            x = fma(p.wx[1], (double)q[1], x);
+            x = fma(p.wx[2], (double)q[2], x);
+            h[r] = fma(p.wx[3], (double)q[3], x);
+        }
+        const double b = geo_b[l];
+#pragma unroll
+        for (int rr = 0; rr < GEO_R; rr++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) v = fma(p.wy[k], h[rr + k], v);
+            acc[rr] = fma(b, v, acc[rr]);
+        }
+    };
+    // the layers outside the last group, then the group (two loops with
+    // straight bodies: no accumulator copies)
+    for (int l = 0; l < L; l++)
+        if (l < lf || l >= lf + lc) layer(l, acc_r);
+    for (int l = lf; l < lf + lc; l++) layer(l, acc_l);
+    if (!colin) return;
+#pragma unroll
+    for (int rr = 0; rr < GEO_R; rr++) {
+        if (ty * GEO_R + rr >= nrows) break;
+        const size_t o = ((size_t)d * N + jf0 + rr) * N + cf0 + t;
+        vr[o] = acc_r[rr];
+        vl[o] = acc_l[rr];
+    }
+}
+
+hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st) {
+    if (a.vr == nullptr || a.vl == nullptr || a.geo_b == nullptr) return hipErrorInvalidValue;
+    const int n = full ? a.N : a.Nout, first = full ? 0 : (a.N - a.Nout) / 2;
+    const long tiles = (long)((n + GEO_W - 1) / GEO_W) * ((n + GEO_R - 1) / GEO_R);
+    if (tiles <= 0) return hipSuccess;
+    if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)tiles, a.D), block(GEO_W);
+    k_geo_planes<<<grid, block, 0, st>>>(a.vr, a.vl, a.all.masters, a.all.ls, a.geo_b, a.N, (int)a.all.M, a.all.L,
+                                         a.last_first, a.last_count, first, n, first, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st) {
     const int n = full ? a.N : a.Nout;
     const long tiles = (long)((n + EXT_W - 1) / EXT_W) * ((n + EXT_R - 1) / EXT_R);
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
     dim3 grid((unsigned)tiles, a.D * ((a.W + 7) / 8)), block(EXT_T);
-    if (full) k_extract<true><<<grid, block, 0, st>>>(a);
-    else k_extract<false><<<grid, block, 0, st>>>(a);
+    if (a.vr == nullptr) {
+        if (full) k_extract<true, 0><<<grid, block, 0, st>>>(a);
+        else k_extract<false, 0><<<grid, block, 0, st>>>(a);
+    } else {
+        if (full) k_extract<true, 2><<<grid, block, 0, st>>>(a);
+        else k_extract<false, 2><<<grid, block, 0, st>>>(a);
+    }
     return hipGetLastError();
 }
 

@@ -46,6 +46,14 @@ struct ExtractArgs {
     int N, Nout, W, D;
     int amplitude;           // 0: phase only (pha = PHI, amp = 1)
     int xs;                  // Nout / 20 blocks per side
+    // factored phase multipliers s_wl = geo_a[w] geo_b[l]: the planes vr, vl
+    // ([D][N][N], FP64) of sum_l geo_b[l] v_l over the layers before the last
+    // group and over the last group, at full-grid positions; vr null: the
+    // extraction samples the layers itself
+    const double* geo_a;     // [W]
+    const double* geo_b;     // [L]
+    double* vr;
+    double* vl;
     int w_begin;             // first field of this chunk of <= 8 wavelengths
 };
 

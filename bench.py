@@ -543,7 +543,7 @@ def main():
         for _ in range(min(args.steps, 20)):
             sim.frame(sync=False)
         ctx.sync()
-    prof = {k: sim.profile_read(k) for k in ("row", "col", "row2", "col2", "phase", "extract", "block")}
+    prof = {k: sim.profile_read(k) for k in ("row", "col", "row2", "col2", "phase", "geometry", "extract", "block")}
     sim.profile(False)
     pcie = None
     if args.host_copy_steps > 0:

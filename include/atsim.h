@@ -90,6 +90,14 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             captured at the second frame (and again when the output
  *             pointers change, or at the second frame after a tuning
  *             change); the position upload stays outside the graph
+ *   "extract_planes" 1 (default): when the phase multipliers factor
+ *             exactly as s_wl = a_w b_l (a_w = s_w0, b_l = s_0l / s_00, as
+ *             the lambda_ref / lambda rows do), the layers' screens are
+ *             summed into two FP64 planes per direction (sum b_l v_l over the
+ *             layers before the last group and over the last group, 16 n^2
+ *             bytes) by a kernel that runs beside the Fresnel passes, and
+ *             the extraction reads them (PHI_w = a_w Vr + a_w Vl) instead of
+ *             sampling every layer; 2: the extraction samples the layers
  * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
 int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
 int atsim_get_tuning(atsim_ctx *ctx, const char *key);
@@ -265,7 +273,8 @@ long atsim_sim_gaps(atsim_sim *sim);
  * with one factor (screen + x factor, or x factor alone), 1 column pass with
  * one factor, 2 extraction, 3 block unwrap, 4 row pass with two x factors
  * and a screen, 5 column pass with two y factors and a screen, 6 sampling of
- * the phase planes that column passes apply, 7 output statistics; returns
+ * the phase planes that column passes apply, 7 output statistics, 8 the
+ * geometric-phase planes of the extraction ("extract_planes"); returns
  * the summed duration [ms] and the number of launches recorded since the
  * last enable. */
 int atsim_sim_profile(atsim_sim *sim, int enable);

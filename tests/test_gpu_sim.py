@@ -139,6 +139,45 @@ def test_layer_dependent_phase_multipliers(ctx, oracle, separable):
     _compare(c, oracle_sims(oracle, c), gpu_sim(ctx, c), nframes=2)
 
 
+@pytest.mark.parametrize("streams", [1, 2])
+def test_extraction_from_geometric_phase_planes(oracle, streams):
+    # factored multipliers (lambda_ref / lambda rows): the extraction reads
+    # the two planes of the geometric phase made beside the passes (tuning
+    # "extract_planes" 1) or samples every layer itself (2); amplitudes are
+    # the same bits, phases agree to FP64 rounding, and both match the oracle
+    from atmosphericturbulencesimul_amd.api import Context
+    lams = [0.5e-6, 0.8e-6, 1.25e-6, 2.2e-6]
+    c = _cfg1_like(oracle, N=128, Nout=100, M=256, lambdas=lams, unwrap_mode=[0, 1, 1, 1])
+    print(f"case: 128^2 -> 100^2, 3 layers, 4 wavelengths, streams {streams}, 3 frames", flush=True)
+    outs = []
+    for mode in (1, 2):
+        cx = Context(0)
+        cx.set_tuning("extract_planes", mode)
+        cx.set_tuning("streams", streams)
+        assert cx.get_tuning("extract_planes") == mode
+        gs = gpu_sim(cx, c)
+        frames = []
+        for _ in range(3):
+            pha, amp = gs.frame()
+            frames.append((pha.cpu().numpy().copy(), amp.cpu().numpy().copy()))
+        gs.profile(True)
+        gs.frame()
+        assert gs.profile_read("geometry")[1] == (1 if mode == 1 else 0)
+        gs.profile(False)
+        outs.append(frames)
+        gs.close()
+        cx.close()
+    for (p1, a1), (p2, a2) in zip(*outs):
+        assert np.array_equal(a1, a2)
+        d = np.abs(p1 - p2)
+        assert int((d > 1.0).sum()) <= 2
+        assert float(d[d <= 1.0].max()) < 1e-5
+    cx = Context(0)
+    cx.set_tuning("streams", streams)
+    _compare(c, oracle_sims(oracle, c), gpu_sim(cx, c), nframes=2)
+    cx.close()
+
+
 def _block_sums_np(E, PHI, Nout):
     """sum cos / sin of PHI - arg E per (jj2, ii2) block over the centred
     Nout x Nout crop, blocks (long)(1.0 * i / Nout * xs) with xs = Nout / 20
