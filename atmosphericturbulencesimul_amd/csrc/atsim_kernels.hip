@@ -1586,12 +1586,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
 // vr / vl [D][N][N] (full-grid positions) = sum_l b_l v_l over the layers
 // outside the last group [lf, lf + lc) and over that group, for full-grid
 // rows r_first .. r_first + nrows - 1 and columns c_first .. + ncols - 1.
-// A workgroup makes 4 rows x 256 columns: per layer it stages the 7 master
-// rows under them (LDS, 7 KB) and each thread makes its column's 7
-// x-filtered values once and its 4 samples (the taps and order of the
-// extraction's sampling).  With a multiple of 8 tile columns each XCD walks
-// its stripe of tile columns down the rows (tiles share 3 master rows).
-constexpr int GEO_W = 256, GEO_R = 4, GEO_SR = GEO_R + 3;
+// A workgroup makes 8 rows x 256 columns: per layer it stages the 11 master
+// rows under them (LDS, 11 KB; the next layer's loads issued before this
+// layer's arithmetic) and each thread makes its column's 11 x-filtered
+// values once and its 8 samples (the taps and order of the extraction's
+// sampling).  With a multiple of 8 tile columns each XCD walks its stripe of
+// tile columns down the rows (tiles share 3 master rows).
+constexpr int GEO_W = 256, GEO_R = 8, GEO_SR = GEO_R + 3;
 __global__ void __launch_bounds__(GEO_W)
 k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __restrict__ masters,
              const LayerSample* __restrict__ ls, const double* __restrict__ geo_b, int N, int M, int L, int lf, int lc,
@@ -1613,25 +1614,36 @@ k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __re
     double acc_r[GEO_R], acc_l[GEO_R];
 #pragma unroll
     for (int q = 0; q < GEO_R; q++) acc_r[q] = acc_l[q] = 0.0;
-    // one layer: stage, then this column's 4 samples into acc
-    auto layer = [&](int l, double(&acc)[GEO_R]) {
+    // layer order: the layers outside the last group, then the group
+    const int nrest = L - lc;
+    auto layer_at = [&](int k) { return k < nrest ? (k < lf ? k : k + lc) : lf + (k - nrest); };
+    // this thread's staged values of one layer (its column of the rows, and
+    // one of the 3 columns right of the tile for t < 3 GEO_SR), loaded one
+    // layer ahead of their use
+    float v[GEO_SR], vt = 0.f;
+    auto fetch = [&](int l) {
         const LayerSample& p = ls[d * L + l];
         const int ys = mod_m(p.y0 + jf0 - 1, M), xs0 = mod_m(p.x0 + cf0 - 1, M);  // uniform
         const float* m = masters + (size_t)l * M * M;
-        __syncthreads();  // the previous layer's staged rows are read
-        {
-            const int x = wrap_col(xs0 + t, M);
-            float v[GEO_SR];
+        const int x = wrap_col(xs0 + t, M);
 #pragma unroll
-            for (int r = 0; r < GEO_SR; r++) v[r] = m[(size_t)wrap_col(ys + r, M) * M + x];
-#pragma unroll
-            for (int r = 0; r < GEO_SR; r++) S[r][t] = v[r];
-            if (t < GEO_SR * 3) {
-                const int r = t / 3, c = GEO_W + (t - r * 3);
-                S[r][c] = m[(size_t)wrap_col(ys + r, M) * M + wrap_col(xs0 + c, M)];
-            }
+        for (int r = 0; r < GEO_SR; r++) v[r] = m[(size_t)wrap_col(ys + r, M) * M + x];
+        if (t < GEO_SR * 3) {
+            const int r = t / 3, c = GEO_W + (t - r * 3);
+            vt = m[(size_t)wrap_col(ys + r, M) * M + wrap_col(xs0 + c, M)];
         }
+    };
+    // stage the fetched layer k, fetch layer k + 1, then this column's
+    // GEO_R samples of layer k into acc
+    auto layer = [&](int k, double(&acc)[GEO_R]) {
+        const int l = layer_at(k);
+        __syncthreads();  // the previous layer's staged rows are read
+#pragma unroll
+        for (int r = 0; r < GEO_SR; r++) S[r][t] = v[r];
+        if (t < GEO_SR * 3) S[t / 3][GEO_W + t % 3] = vt;
         __syncthreads();
+        if (k + 1 < L) fetch(layer_at(k + 1));
+        const LayerSample& p = ls[d * L + l];
         double h[GEO_SR];
 #pragma unroll
         for (int r = 0; r < GEO_SR; r++) {
@@ -1644,17 +1656,16 @@ k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __re
         const double b = geo_b[l];
 #pragma unroll
         for (int rr = 0; rr < GEO_R; rr++) {
-            double v = 0.0;
+            double s = 0.0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) v = fma(p.wy[k], h[rr + k], v);
-            acc[rr] = fma(b, v, acc[rr]);
+            for (int q = 0; q < 4; q++) s = fma(p.wy[q], h[rr + q], s);
+            acc[rr] = fma(b, s, acc[rr]);
         }
     };
-    // the layers outside the last group, then the group (two loops with
-    // straight bodies: no accumulator copies)
-    for (int l = 0; l < L; l++)
-        if (l < lf || l >= lf + lc) layer(l, acc_r);
-    for (int l = lf; l < lf + lc; l++) layer(l, acc_l);
+    if (L > 0) fetch(layer_at(0));
+    // two loops with straight bodies: no accumulator copies
+    for (int k = 0; k < nrest; k++) layer(k, acc_r);
+    for (int k = nrest; k < L; k++) layer(k, acc_l);
     if (!colin) return;
 #pragma unroll
     for (int rr = 0; rr < GEO_R; rr++) {
