@@ -39,6 +39,9 @@ struct atsim_ctx {
     // two planes beside the Fresnel passes and the extraction reads them;
     // 2: the extraction samples the layers itself
     int extract_planes = 1;
+    // 1: the block rule's final update visits only the blocks whose largest
+    // |r| can give a non-zero count; 2: every pixel of every block
+    int block_apply = 1;
     unsigned tuning_gen = 0;  // bumped by every accepted atsim_set_tuning: captured frame graphs are stale
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
@@ -350,6 +353,9 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else if (!strcmp(key, "graph")) {
         if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "graph must be 0 or 1");
         c->graph = value;
+    } else if (!strcmp(key, "block_apply")) {
+        if (value < 0 || value > 2) return fail(c, ATSIM_EINVAL, "block_apply must be 0, 1 or 2");
+        c->block_apply = value == 0 ? 1 : value;
     } else if (!strcmp(key, "extract_planes")) {
         if (value < 0 || value > 2) return fail(c, ATSIM_EINVAL, "extract_planes must be 0, 1 or 2");
         c->extract_planes = value == 0 ? 1 : value;
@@ -370,6 +376,7 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!strcmp(key, "col2_e")) return c->col2_e;
     if (!strcmp(key, "graph")) return c->graph;
     if (!strcmp(key, "extract_planes")) return c->extract_planes;
+    if (!strcmp(key, "block_apply")) return c->block_apply;
     return ATSIM_EINVAL;
 }
 
@@ -853,7 +860,9 @@ struct atsim_sim {
     hipEvent_t ev_fork = nullptr, ev_phase = nullptr, ev_first[MAX_SETS - 1] = {}, ev_join[MAX_SETS - 1] = {};
     float* dres = nullptr;   // [F][Nout][Nout] block-rule remainders
     int* blk = nullptr;      // [Nout] block index of each output column / row
-    double* bsum = nullptr;
+    double* bsum = nullptr;   // [F][nb][2] block sums, then rmax in the same allocation
+    unsigned* rmax = nullptr; // [F][nb] bits of each block's largest |r| (float)
+    int* bstart = nullptr;    // [xs + 1] first output row / column of each block, then Nout
     double* bin = nullptr;
     double* ch = nullptr;
     unsigned* cmax = nullptr;  // [F] largest first-round count of each block-rule field
@@ -972,6 +981,7 @@ static void sim_free(atsim_sim* s) {
     if (s->ev_phase) hipEventDestroy(s->ev_phase);
     hipFree(s->blk);
     hipFree(s->bsum);
+    hipFree(s->bstart);
     hipFree(s->bin);
     hipFree(s->ch);
     hipFree(s->cmax);
@@ -985,6 +995,10 @@ static void sim_free(atsim_sim* s) {
 }
 
 extern "C" {
+
+// doubles of the block-sum allocation: F nb (cos, sin) sums, then F nb
+// unsigned maxima
+static long bsum_doubles(long F, long nb) { return F * nb * 2 + (F * nb + 1) / 2; }
 
 int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (!c || !d || !out) return ATSIM_EINVAL;
@@ -1102,7 +1116,10 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (s->nblockfields > 0) {
         const size_t nb = (size_t)(s->xs > 0 ? s->xs * s->xs : 1);
         SIMALLOC(s->dres, sizeof(float) * F * no);
-        SIMALLOC(s->bsum, sizeof(double) * F * nb * 2);
+        // block sums and largest |r|, cleared together every frame
+        SIMALLOC(s->bsum, sizeof(double) * bsum_doubles(F, nb));
+        if (s->bsum) s->rmax = reinterpret_cast<unsigned*>(s->bsum + F * nb * 2);
+        SIMALLOC(s->bstart, sizeof(int) * (s->xs + 1));
         SIMALLOC(s->bin, sizeof(double) * F * nb);
         SIMALLOC(s->ch, sizeof(double) * F * nb);
         SIMALLOC(s->cmax, sizeof(unsigned) * F);
@@ -1123,6 +1140,11 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     // block of output column / row i: (long)(1.0 * i / Nout * xs) (AT.c:3611)
     std::vector<int> blk(s->Nout > 0 ? s->Nout : 1, 0);
     for (long i = 0; i < s->Nout; i++) blk[i] = (int)((double)i / s->Nout * s->xs);
+    // blk is non-decreasing in i: block b covers bstart[b] .. bstart[b + 1] - 1
+    std::vector<int> bstart((size_t)s->xs + 1, (int)s->Nout);
+    for (long i = s->Nout - 1; i >= 0; i--)
+        if (blk[i] < s->xs) bstart[blk[i]] = (int)i;
+    for (long b = s->xs - 1; b >= 0; b--) bstart[b] = std::min(bstart[b], bstart[b + 1]);
     if (r == ATSIM_OK) {
         if (copy_on_stream(c, s->s_dev, s->scoeff.data(), sizeof(double) * s->scoeff.size()) != hipSuccess ||
             (s->factored && (copy_on_stream(c, s->geo_a, geo_a.data(), sizeof(double) * s->W) != hipSuccess ||
@@ -1130,7 +1152,8 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
             copy_on_stream(c, s->unwrap_dev, s->unwrap.data(), sizeof(int) * s->W) != hipSuccess ||
             copy_on_stream(c, s->blk, blk.data(), sizeof(int) * blk.size()) != hipSuccess ||
             (s->nblockfields > 0 &&
-             copy_on_stream(c, s->flist_dev, flist.data(), sizeof(int) * flist.size()) != hipSuccess) ||
+             (copy_on_stream(c, s->flist_dev, flist.data(), sizeof(int) * flist.size()) != hipSuccess ||
+              copy_on_stream(c, s->bstart, bstart.data(), sizeof(int) * bstart.size()) != hipSuccess)) ||
             (s->nslots > 0 &&
              copy_on_stream(c, s->slot_layer_dev, slot_layer.data(), sizeof(int) * slot_layer.size()) != hipSuccess))
             r = fail(c, ATSIM_EHIP, "sim: parameter upload failed");
@@ -1309,6 +1332,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     a.dres = s->dres;
     a.blk = s->blk;
     a.bsum = s->bsum;
+    a.rmax = (c->block_apply == 1 && s->xs > 0) ? s->rmax : nullptr;
     a.unwrap_mode = s->unwrap_dev;
     a.all = sa;
     a.all.wlo = a.all.wcnt = 0;
@@ -1396,15 +1420,20 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         HIPCHK(c, launch_extract_geometry(a, full, ms));
         prof_end(s, pg);
     }
-    if (s->nblockfields > 0 && s->xs > 0) HIPCHK(c, launch_zero_f64(s->bsum, F * s->xs * s->xs * 2, ms));
+    if (s->nblockfields > 0 && s->xs > 0)
+        HIPCHK(c, launch_zero_f64(s->bsum, bsum_doubles(F, (long)s->xs * s->xs), ms));
     long pe = prof_begin(s, PROF_EXTRACT);
     HIPCHK(c, launch_extract(a, full, ms));
     prof_end(s, pe);
     if (s->nblockfields > 0) {
         long pb = prof_begin(s, PROF_BLOCK);
         HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->cmax, s->flist_dev, s->nblockfields, s->xs, ms));
-        HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields, (int)s->Nout,
-                                     s->xs, ms));
+        if (a.rmax)
+            HIPCHK(c, launch_block_apply_sparse(d_pha, s->dres, s->bin, s->rmax, s->flist_dev, s->bstart,
+                                                s->nblockfields, (int)s->Nout, s->xs, ms));
+        else
+            HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields,
+                                         (int)s->Nout, s->xs, ms));
         prof_end(s, pb);
     }
     if (s->stats) {

@@ -1321,6 +1321,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     __shared__ float S[EXT_LC][EXT_SR][EXT_SW];   // staged master rows
     __shared__ double H[EXT_LC][EXT_SR][EXT_W];   // their x-filtered values
     __shared__ double bs[2][8][2][EXT_BC];        // [block row - jb0][w][cos, sin][block column - b0]
+    __shared__ unsigned bm[2][8][EXT_BC];         // bits of the block's largest |r|, same indices
     __shared__ LayerSample sls[MAX_LAYERS];       // this direction's layer samples
     __shared__ double ss[MAX_LAYERS][8];          // phase scales of this chunk's wavelengths (0 past Wc)
     __shared__ int ysq[MAX_LAYERS], xsq[MAX_LAYERS];  // wrapped first staged row / column per layer
@@ -1388,6 +1389,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     if (OUTPUT) {
         if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
         for (int k = threadIdx.x; k < 2 * 8 * 2 * EXT_BC; k += EXT_T) (&bs[0][0][0][0])[k] = 0.0;
+        for (int k = threadIdx.x; k < 2 * 8 * EXT_BC; k += EXT_T) (&bm[0][0][0])[k] = 0u;
     }
     __syncthreads();
     // the fields are needed only after the layers: issue their loads first
@@ -1536,13 +1538,18 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                         const double n = rint(dd * inv2pi);
                         const double r = fma(-6.283185307179586477, n, dd);
                         a.pha[oi] = (float)fma(6.283185307179586477, n, ph);
-                        if (a.dres) a.dres[oi] = (float)r;
+                        const float rf = (float)r;
+                        if (a.dres) a.dres[oi] = rf;
                         if (ii2 >= 0) {
                             // contribution to the block's mean direction,
                             // hardware sin/cos (the block sums only feed the
                             // atan2 of a 400-pixel sum)
-                            cx = __cosf((float)r);
-                            cy = __sinf((float)r);
+                            cx = __cosf(rf);
+                            cy = __sinf(rf);
+                            // the block's largest |r| (the bits of a
+                            // non-negative float order like the float)
+                            if (a.rmax && ii2 - b0 < EXT_BC)
+                                atomicMax(&bm[jb][w][ii2 - b0], __float_as_uint(fabsf(rf)));
                         }
                     }
                 }
@@ -1579,6 +1586,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         double* dst = a.bsum + ((size_t)f * nb + (size_t)(jb0 + j) * xs + b0 + lb) * 2;
         atomicAdd(dst, bs[j][w][0][lb]);
         atomicAdd(dst + 1, bs[j][w][1][lb]);
+    }
+    if (a.rmax == nullptr) return;
+    for (int k = threadIdx.x; k < 2 * 8 * EXT_BC; k += EXT_T) {
+        const int j = k / (8 * EXT_BC), w = (k / EXT_BC) & 7, lb = k % EXT_BC;
+        if (w >= Wc || um[w] == 0 || jb0 + j >= xs || b0 + lb >= xs || bm[j][w][lb] == 0u) continue;
+        const int f = d * a.W + w_begin + w;
+        atomicMax(a.rmax + (size_t)f * nb + (size_t)(jb0 + j) * xs + b0 + lb, bm[j][w][lb]);
     }
 }
 
@@ -1933,6 +1947,54 @@ __global__ void __launch_bounds__(256) k_block_apply(float* __restrict__ pha, co
             if (n != 0.0) prow[ii] = (float)fma(2.0 * M_PI, n, (double)prow[ii]);
         }
     }
+}
+
+// The same update where it can change a phase: a block's pixels all keep
+// their phase when the counts of -R and R are zero, R the largest |r| of the
+// block's stored remainders (the count is non-decreasing in r), so one lane
+// per block tests that from the extraction's R, and the wave then visits
+// only the blocks that fail it (pixels as apply_count, in any order: each is
+// independent).  bstart[b] is the first output row / column of block b,
+// bstart[xs] = Nout.
+__global__ void __launch_bounds__(64) k_block_apply_sparse(float* __restrict__ pha, const float* __restrict__ dres,
+                                                           const double* __restrict__ bin,
+                                                           const unsigned* __restrict__ rmax,
+                                                           const int* __restrict__ flist,
+                                                           const int* __restrict__ bstart, int Nout, int xs) {
+    const int f = flist[blockIdx.y];
+    const int nb = xs * xs;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x * 64 + lane;
+    bool need = false;
+    if (b < nb) {
+        const float R = __uint_as_float(rmax[(size_t)f * nb + b]);
+        const double peak = bin[(size_t)f * nb + b];
+        need = apply_count(-R, peak) != 0.0 || apply_count(R, peak) != 0.0;
+    }
+    unsigned long long m = __ballot(need);
+    while (m) {
+        const int bb = blockIdx.x * 64 + (__ffsll((long long)m) - 1);
+        m &= m - 1;
+        const int by = bb / xs, bx = bb - by * xs;
+        const int r0 = bstart[by], c0 = bstart[bx];
+        const int w = bstart[bx + 1] - c0, n = (bstart[by + 1] - r0) * w;
+        const double peak = bin[(size_t)f * nb + bb];
+        for (int e = lane; e < n; e += 64) {
+            const int q = e / w;
+            const size_t o = ((size_t)f * Nout + r0 + q) * Nout + c0 + (e - q * w);
+            const double cnt = apply_count(dres[o], peak);
+            if (cnt != 0.0) pha[o] = (float)fma(2.0 * M_PI, cnt, (double)pha[o]);
+        }
+    }
+}
+
+hipError_t launch_block_apply_sparse(float* pha, const float* dres, const double* bin, const unsigned* rmax,
+                                     const int* field_list, const int* bstart, int nfields, int Nout, int xs,
+                                     hipStream_t st) {
+    if (nfields <= 0 || xs <= 0) return hipSuccess;
+    dim3 grid((xs * xs + 63) / 64, nfields);
+    k_block_apply_sparse<<<grid, 64, 0, st>>>(pha, dres, bin, rmax, field_list, bstart, Nout, xs);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,

@@ -40,6 +40,7 @@ struct ExtractArgs {
     float* dres;             // [F][Nout][Nout] (or null)
     const int* blk;          // [Nout] block index (long)(1.0 * i / Nout * xs) of column / row i
     double* bsum;            // [F][nb][2] block sums of cos/sin(PHI - atan2)
+    unsigned* rmax;          // [F][nb] bits of each block's largest |r| (float), or null
     const int* unwrap_mode;  // [W]
     ScreenArgs all;          // every layer: the geometric phase PHI
     int last_first, last_count;  // super-layer group whose screen is applied here

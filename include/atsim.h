@@ -98,6 +98,10 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             bytes) by a kernel that runs beside the Fresnel passes, and
  *             the extraction reads them (PHI_w = a_w Vr + a_w Vl) instead of
  *             sampling every layer; 2: the extraction samples the layers
+ *   "block_apply" 1 (default): the final update of the block unwrap rule
+ *             visits only the blocks where a count can be non-zero (from
+ *             each block's largest |r|, kept by the extraction); 2: every
+ *             pixel.  Both give the same bits.
  * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
 int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
 int atsim_get_tuning(atsim_ctx *ctx, const char *key);

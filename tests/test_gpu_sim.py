@@ -259,17 +259,26 @@ def test_block_rule_in_strong_turbulence(ctx, oracle, r0_m):
     # strong turbulence at 0.5 and 0.8 um, both on the block rule: the oracle
     # finds about 160 (r0 8 cm) and 3 100 (4 cm) pixels per frame whose final
     # block count is not zero (the final phase is not arg E + 2 pi rint((PHI -
-    # arg E) / 2 pi)); the GPU's phases match the oracle's on them too
+    # arg E) / 2 pi)); the GPU's phases match the oracle's on them too, and
+    # the final update that visits only the blocks where a count can be
+    # non-zero (tuning "block_apply" 1, the default) gives the bits of the
+    # update over every pixel (2)
+    from atmosphericturbulencesimul_amd.api import Context
     c = build_case(oracle, N=128, Nout=100, M=256, alts=[0.0, 4000.0, 12000.0], cn2=[0.5, 0.3, 0.2],
                    speeds=[10.0, 20.0, 30.0], dirs=np.radians([0.0, 90.0, 200.0]), lambdas=[0.5e-6, 0.8e-6],
                    lam_ref=0.5e-6, dx=0.02, dt=0.001, r0_m=r0_m, L0=20.0, l0=0.01, unwrap_mode=[1, 1])
     print(f"case: 128^2 -> 100^2, 3 layers, r0 {r0_m} m, block rule at 0.5 / 0.8 um, 3 frames", flush=True)
+    cx = Context(0)
+    cx.set_tuning("block_apply", 2)
+    assert cx.get_tuning("block_apply") == 2 and ctx.get_tuning("block_apply") == 1
+    gd = gpu_sim(cx, c)
     gs = gpu_sim(ctx, c)
     os_ = oracle_sims(oracle, c)[0]
     o = (c.N - c.Nout) // 2
     counted = 0
     for _ in range(3):
         pha_g = gs.frame()[0].cpu().numpy()
+        assert np.array_equal(pha_g, gd.frame()[0].cpu().numpy())
         E, PHI, pha_o, _ = os_.frame()
         for w in range(c.W):
             Ec, Pc = E[w][o:o + c.Nout, o:o + c.Nout], PHI[w][o:o + c.Nout, o:o + c.Nout]
@@ -282,6 +291,8 @@ def test_block_rule_in_strong_turbulence(ctx, oracle, r0_m):
     print("pixels with a non-zero count in the oracle:", counted, flush=True)
     assert counted >= 300
     gs.close()
+    gd.close()
+    cx.close()
 
 
 @pytest.mark.parametrize("streams,col_e", [(2, 16), (2, 32), (3, 16), (4, 16)])
