@@ -1330,6 +1330,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     a.pha = d_pha;
     a.amp = d_amp;
     a.dres = s->dres;
+    RemainderSource rsrc{};
     a.blk = s->blk;
     a.bsum = s->bsum;
     a.rmax = (c->block_apply == 1 && s->xs > 0) ? s->rmax : nullptr;
@@ -1362,6 +1363,14 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         a.vl = s->vgeo + (size_t)D * s->N * s->N;
     }
     bool geometry_done = false;
+    // the sparse final update of the block rule makes the remainders again
+    // from the final fields and the planes (the extraction then writes no
+    // map); without propagation there are no fields (the field is 1)
+    if (planes && !full && a.rmax != nullptr && s->propagate) {
+        a.dres = nullptr;
+        rsrc = RemainderSource{s->fields, a.vr, a.vl, s->geo_a, (int)s->N, (int)W, (int)(s->N - s->Nout) / 2,
+                               a.amplitude && a.last_count > 0};
+    }
     auto phase = [&](hipStream_t st) {
         return launch_layer_phase(s->vq, s->masters, s->ls_dev, s->slot_layer_dev, s->nslots, s->N, M, L, D, st);
     };
@@ -1429,8 +1438,8 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         long pb = prof_begin(s, PROF_BLOCK);
         HIPCHK(c, launch_block_fix(s->bsum, s->bin, s->ch, s->cmax, s->flist_dev, s->nblockfields, s->xs, ms));
         if (a.rmax)
-            HIPCHK(c, launch_block_apply_sparse(d_pha, s->dres, s->bin, s->rmax, s->flist_dev, s->bstart,
-                                                s->nblockfields, (int)s->Nout, s->xs, ms));
+            HIPCHK(c, launch_block_apply_sparse(d_pha, a.dres, s->bin, s->rmax, s->flist_dev, s->bstart,
+                                                s->nblockfields, (int)s->Nout, s->xs, rsrc, ms));
         else
             HIPCHK(c, launch_block_apply(d_pha, s->dres, s->bin, s->flist_dev, s->blk, s->nblockfields,
                                          (int)s->Nout, s->xs, ms));

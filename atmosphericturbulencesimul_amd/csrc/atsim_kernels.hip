@@ -1255,6 +1255,8 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 // spacing near pi is 2.4e-7; float atan2 itself errs by up to 3.4e-7 on
 // random Gaussian arguments); (0, 0) -> 0.
 __device__ __forceinline__ float atan2_poly(float y, float x) {
+    // no contracted products: every kernel that inlines it gets the same bits
+#pragma clang fp contract(off)
     const float ax = fabsf(x), ay = fabsf(y);
     const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
     const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
@@ -1271,6 +1273,31 @@ __device__ __forceinline__ float atan2_poly(float y, float x) {
     if (ay > ax) r = 1.5707963267948966f - r;
     if (x < 0.f) r = 3.141592653589793f - r;
     return copysignf(r, y);
+}
+
+// The steps of the extraction that the block rule's final update repeats
+// for the blocks it visits when the remainders are not stored; written
+// without contracted products so that both kernels get the same bits.
+// Factored geometric phase of one wavelength from the planes: the last
+// group's part PL = a vl and the total PHI = a vr + PL.
+__device__ __forceinline__ void planes_phase(double a, double vr, double vl, double& phi, double& pl) {
+#pragma clang fp contract(off)
+    pl = a * vl;
+    phi = fma(a, vr, pl);
+}
+// arg of the field rotated by the last group's screen, not yet in (-pi, pi]
+__device__ __forceinline__ double rotated_arg(float2 E, double rot) {
+#pragma clang fp contract(off)
+    return (double)atan2_poly(E.y, E.x) + rot;
+}
+// the block rule's whole turns n = rint((PHI - ph) / 2 pi) and FP64
+// remainder r = PHI - ph - 2 pi n
+__device__ __forceinline__ double block_turns(double phi, double ph, double& r) {
+#pragma clang fp contract(off)
+    const double dd = phi - ph;
+    const double n = rint(dd * 0.15915494309189533577);
+    r = fma(-6.283185307179586477, n, dd);
+    return n;
 }
 
 // Inclusive prefix sum over the 64 lanes of a wave: shifts by 1, 2, 4, 8
@@ -1407,11 +1434,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     if (MODE == 2 && valid) {
         const double vr = a.vr[gi], vl = a.vl[gi];
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-            const double aw = w < Wc ? a.geo_a[w_begin + w] : 0.0;
-            PHI[w] = aw * vr;
-            PL[w] = aw * vl;
-        }
+        for (int w = 0; w < 8; w++) planes_phase(w < Wc ? a.geo_a[w_begin + w] : 0.0, vr, vl, PHI[w], PL[w]);
     }
     for (int l0 = 0; SAMPLE && l0 < L; l0 += EXT_LC) {
         const int nl = min(EXT_LC, L - l0);
@@ -1478,8 +1501,11 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
         }
     }
     // the total geometric phase: the other layers' sum plus the last group's
+    // (from the planes it is the total already)
+    if (MODE == 0) {
 #pragma unroll
-    for (int w = 0; w < 8; w++) PHI[w] += PL[w];
+        for (int w = 0; w < 8; w++) PHI[w] += PL[w];
+    }
     const double inv2pi = 0.15915494309189533577;
     // block column of lane + 1 (-1: none, or the last lane) and whether this
     // lane ends a run of one block column
@@ -1522,7 +1548,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                     // the polynomial arctangent (< 4e-7 rad)
                     a.amp[oi] = __builtin_amdgcn_sqrtf(E.x * E.x + E.y * E.y);
                     // arg of the rotated field, not yet brought into (-pi, pi]
-                    const double ph = (double)atan2_poly(E.y, E.x) + rot;
+                    const double ph = rotated_arg(E, rot);
                     if (!block_rule) {
                         // this rule adds whole turns of PHI alone to the phase
                         // in (-pi, pi] (AT.c:3456), so the rotated phase is
@@ -1534,9 +1560,8 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                         // FP64 remainder (rounded to float) waits for the
                         // block's peak; the final count of the block rule is
                         // n + count((r - peak) / 2 pi)
-                        const double dd = PHI[w] - ph;
-                        const double n = rint(dd * inv2pi);
-                        const double r = fma(-6.283185307179586477, n, dd);
+                        double r;
+                        const double n = block_turns(PHI[w], ph, r);
                         a.pha[oi] = (float)fma(6.283185307179586477, n, ph);
                         const float rf = (float)r;
                         if (a.dres) a.dres[oi] = rf;
@@ -1960,7 +1985,8 @@ __global__ void __launch_bounds__(64) k_block_apply_sparse(float* __restrict__ p
                                                            const double* __restrict__ bin,
                                                            const unsigned* __restrict__ rmax,
                                                            const int* __restrict__ flist,
-                                                           const int* __restrict__ bstart, int Nout, int xs) {
+                                                           const int* __restrict__ bstart, int Nout, int xs,
+                                                           RemainderSource rs) {
     const int f = flist[blockIdx.y];
     const int nb = xs * xs;
     const int lane = threadIdx.x;
@@ -1980,9 +2006,23 @@ __global__ void __launch_bounds__(64) k_block_apply_sparse(float* __restrict__ p
         const int w = bstart[bx + 1] - c0, n = (bstart[by + 1] - r0) * w;
         const double peak = bin[(size_t)f * nb + bb];
         for (int e = lane; e < n; e += 64) {
-            const int q = e / w;
-            const size_t o = ((size_t)f * Nout + r0 + q) * Nout + c0 + (e - q * w);
-            const double cnt = apply_count(dres[o], peak);
+            const int q = e / w, jj = r0 + q, ii = c0 + (e - q * w);
+            const size_t o = ((size_t)f * Nout + jj) * Nout + ii;
+            float rf;
+            if (rs.fields) {
+                // the extraction's remainder again, from the final field and
+                // the geometric phase's planes at the full-grid pixel
+                const int dd = f / rs.W, jf = jj + rs.o, ifl = ii + rs.o;
+                const size_t gi = ((size_t)dd * rs.N + jf) * rs.N + ifl;
+                double phi, pl, r;
+                planes_phase(rs.geo_a[f - dd * rs.W], rs.vr[gi], rs.vl[gi], phi, pl);
+                const float2 E = rs.fields[((size_t)f * rs.N + jf) * rs.N + ifl];
+                block_turns(phi, rotated_arg(E, rs.rot ? pl : 0.0), r);
+                rf = (float)r;
+            } else {
+                rf = dres[o];
+            }
+            const double cnt = apply_count(rf, peak);
             if (cnt != 0.0) pha[o] = (float)fma(2.0 * M_PI, cnt, (double)pha[o]);
         }
     }
@@ -1990,10 +2030,11 @@ __global__ void __launch_bounds__(64) k_block_apply_sparse(float* __restrict__ p
 
 hipError_t launch_block_apply_sparse(float* pha, const float* dres, const double* bin, const unsigned* rmax,
                                      const int* field_list, const int* bstart, int nfields, int Nout, int xs,
-                                     hipStream_t st) {
+                                     const RemainderSource& rs, hipStream_t st) {
     if (nfields <= 0 || xs <= 0) return hipSuccess;
+    if (rs.fields == nullptr && dres == nullptr) return hipErrorInvalidValue;
     dim3 grid((xs * xs + 63) / 64, nfields);
-    k_block_apply_sparse<<<grid, 64, 0, st>>>(pha, dres, bin, rmax, field_list, bstart, Nout, xs);
+    k_block_apply_sparse<<<grid, 64, 0, st>>>(pha, dres, bin, rmax, field_list, bstart, Nout, xs, rs);
     return hipGetLastError();
 }
 

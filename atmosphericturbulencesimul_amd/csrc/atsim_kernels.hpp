@@ -58,4 +58,18 @@ struct ExtractArgs {
     int w_begin;             // first field of this chunk of <= 8 wavelengths
 };
 
+// Where the block rule's final update takes the remainders r of the blocks
+// it visits: fields null, the extraction's FP32 map; otherwise they are made
+// again from the final fields [F][N][N] and the factored geometric phase's
+// planes vr, vl [D][N][N] with multipliers geo_a [W] (the output crop starts
+// at full-grid row / column o), as the extraction made them; rot: the
+// extraction rotates by the last group's screen (it has layers).
+struct RemainderSource {
+    const float2* fields;
+    const double* vr;
+    const double* vl;
+    const double* geo_a;
+    int N, W, o, rot;
+};
+
 }  // namespace atsim

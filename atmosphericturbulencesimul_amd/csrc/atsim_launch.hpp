@@ -97,11 +97,12 @@ hipError_t launch_block_counts(const double* bin, float* ch, unsigned* cmax, con
 hipError_t launch_block_apply(float* pha, const float* dres, const double* bin, const int* field_list,
                               const int* blk, int nfields, int Nout, int xs, hipStream_t st);
 // the same update, visiting only the blocks whose largest |r| (rmax, [F][nb]
-// float bits from the extraction) lets a count be non-zero; bstart[b]: first
-// output row / column of block b (xs + 1 entries)
+// float bits from the extraction) lets a count be non-zero, with r from dres
+// or made again as rs says; bstart[b]: first output row / column of block b
+// (xs + 1 entries)
 hipError_t launch_block_apply_sparse(float* pha, const float* dres, const double* bin, const unsigned* rmax,
                                      const int* field_list, const int* bstart, int nfields, int Nout, int xs,
-                                     hipStream_t st);
+                                     const RemainderSource& rs, hipStream_t st);
 
 hipError_t launch_zero_f64(double* p, long n, hipStream_t st);
 // per-field sums of pha, pha^2, amp^2, amp^4 over n pixels of F fields:
