@@ -1385,14 +1385,21 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         // extraction's geometric-phase planes follow on that stream.
         HIPCHK(c, hipEventRecord(s->ev_fork, ms));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
-        if (fused) {
-            HIPCHK(c, phase(s->xs_stream[0]));
+        if (fused && planes) {
+            // the phase planes and the geometric-phase planes in one sweep
+            // over the layers (joins the context stream with set 1's chain)
+            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], s->vq, s->slot_layer_dev, s->nslots));
             HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
-        }
-        if (planes) {
-            // joins the context stream with set 1's chain
-            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0]));
             geometry_done = true;
+        } else {
+            if (fused) {
+                HIPCHK(c, phase(s->xs_stream[0]));
+                HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
+            }
+            if (planes) {
+                HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0]));
+                geometry_done = true;
+            }
         }
         for (long step = 0; step < nsteps; step++) {
             for (int k = 0; k < nsets; k++) {
@@ -1413,7 +1420,12 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
             HIPCHK(c, hipStreamWaitEvent(ms, s->ev_join[k - 1], 0));
         }
     } else if (s->propagate) {
-        if (fused) {
+        if (fused && planes) {
+            long pg = prof_begin(s, PROF_GEOMETRY);
+            HIPCHK(c, launch_extract_geometry(a, full, ms, s->vq, s->slot_layer_dev, s->nslots));
+            prof_end(s, pg);
+            geometry_done = true;
+        } else if (fused) {
             long pp = prof_begin(s, PROF_PHASE);
             HIPCHK(c, phase(ms));
             prof_end(s, pp);

@@ -1621,7 +1621,10 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     }
 }
 
-// The factored geometric phase's planes for the extraction's MODE 2:
+// The factored geometric phase's planes for the extraction's MODE 2, and
+// optionally (vq) the column passes' phase planes of the layers slot_layer
+// in the same sweep (k_layer_phase's values: the same samples in FP64
+// quarter turns), over the whole grid:
 // vr / vl [D][N][N] (full-grid positions) = sum_l b_l v_l over the layers
 // outside the last group [lf, lf + lc) and over that group, for full-grid
 // rows r_first .. r_first + nrows - 1 and columns c_first .. + ncols - 1.
@@ -1635,7 +1638,8 @@ constexpr int GEO_W = 256, GEO_R = 8, GEO_SR = GEO_R + 3;
 __global__ void __launch_bounds__(GEO_W)
 k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __restrict__ masters,
              const LayerSample* __restrict__ ls, const double* __restrict__ geo_b, int N, int M, int L, int lf, int lc,
-             int r_first, int nrows, int c_first, int ncols) {
+             int r_first, int nrows, int c_first, int ncols, double* __restrict__ vq,
+             const int* __restrict__ slot_layer, int nslots) {
     __shared__ float S[GEO_SR][GEO_W + 4];
     const int ntx = (ncols + GEO_W - 1) / GEO_W;
     int tx, ty;
@@ -1650,9 +1654,9 @@ k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __re
     const int d = blockIdx.y, t = threadIdx.x;
     const int jf0 = r_first + ty * GEO_R, cf0 = c_first + tx * GEO_W;
     const bool colin = tx * GEO_W + t < ncols;
-    double acc_r[GEO_R], acc_l[GEO_R];
+    double acc[GEO_R];
 #pragma unroll
-    for (int q = 0; q < GEO_R; q++) acc_r[q] = acc_l[q] = 0.0;
+    for (int q = 0; q < GEO_R; q++) acc[q] = 0.0;
     // layer order: the layers outside the last group, then the group
     const int nrest = L - lc;
     auto layer_at = [&](int k) { return k < nrest ? (k < lf ? k : k + lc) : lf + (k - nrest); };
@@ -1693,37 +1697,50 @@ k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __re
             h[r] = fma(p.wx[3], (double)q[3], x);
         }
         const double b = geo_b[l];
+        // the phase plane of a layer that a column pass applies, if this
+        // launch makes them too (uniform)
+        int slot = -1;
+        for (int q = 0; vq != nullptr && q < nslots; q++)
+            if (slot_layer[q] == l) slot = q;
+        double* pq = slot >= 0 ? vq + (((size_t)slot * gridDim.y + d) * N + jf0) * N + cf0 + t : nullptr;
 #pragma unroll
         for (int rr = 0; rr < GEO_R; rr++) {
             double s = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; q++) s = fma(p.wy[q], h[rr + q], s);
             acc[rr] = fma(b, s, acc[rr]);
+            if (pq != nullptr && colin && ty * GEO_R + rr < nrows) pq[(size_t)rr * N] = s * 0.63661977236758134308;
+        }
+    };
+    // the sums of this column's rows into plane pl, then restarted
+    auto put = [&](double* pl) {
+#pragma unroll
+        for (int rr = 0; rr < GEO_R; rr++) {
+            if (colin && ty * GEO_R + rr < nrows) pl[((size_t)d * N + jf0 + rr) * N + cf0 + t] = acc[rr];
+            acc[rr] = 0.0;
         }
     };
     if (L > 0) fetch(layer_at(0));
-    // two loops with straight bodies: no accumulator copies
-    for (int k = 0; k < nrest; k++) layer(k, acc_r);
-    for (int k = nrest; k < L; k++) layer(k, acc_l);
-    if (!colin) return;
-#pragma unroll
-    for (int rr = 0; rr < GEO_R; rr++) {
-        if (ty * GEO_R + rr >= nrows) break;
-        const size_t o = ((size_t)d * N + jf0 + rr) * N + cf0 + t;
-        vr[o] = acc_r[rr];
-        vl[o] = acc_l[rr];
-    }
+    // the layers before the last group, then the group, one accumulator set
+    for (int k = 0; k < nrest; k++) layer(k, acc);
+    put(vr);
+    for (int k = nrest; k < L; k++) layer(k, acc);
+    put(vl);
 }
 
-hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st) {
+hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st, double* vq,
+                                   const int* slot_layer, int nslots) {
     if (a.vr == nullptr || a.vl == nullptr || a.geo_b == nullptr) return hipErrorInvalidValue;
-    const int n = full ? a.N : a.Nout, first = full ? 0 : (a.N - a.Nout) / 2;
+    if (vq != nullptr && (slot_layer == nullptr || nslots <= 0)) return hipErrorInvalidValue;
+    // the column passes' phase planes cover the whole grid
+    const bool whole = full || vq != nullptr;
+    const int n = whole ? a.N : a.Nout, first = whole ? 0 : (a.N - a.Nout) / 2;
     const long tiles = (long)((n + GEO_W - 1) / GEO_W) * ((n + GEO_R - 1) / GEO_R);
     if (tiles <= 0) return hipSuccess;
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
     dim3 grid((unsigned)tiles, a.D), block(GEO_W);
     k_geo_planes<<<grid, block, 0, st>>>(a.vr, a.vl, a.all.masters, a.all.ls, a.geo_b, a.N, (int)a.all.M, a.all.L,
-                                         a.last_first, a.last_count, first, n, first, n);
+                                         a.last_first, a.last_count, first, n, first, n, vq, slot_layer, nslots);
     return hipGetLastError();
 }
 

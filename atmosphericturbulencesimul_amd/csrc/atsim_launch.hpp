@@ -84,9 +84,11 @@ hipError_t launch_cmul(float2* a, const float2* b, long n, long nbatch, hipStrea
 hipError_t launch_amp_pha(const float2* e, float* amp, float* pha, long n, hipStream_t st);
 hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, hipStream_t st);
 
-// the planes a.vr, a.vl of the factored geometric phase (extraction tiles,
-// one launch index per tile and direction)
-hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st);
+// the planes a.vr, a.vl of the factored geometric phase, and with vq the
+// column passes' phase planes of the nslots layers slot_layer as
+// launch_layer_phase makes them
+hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st, double* vq = nullptr,
+                                   const int* slot_layer = nullptr, int nslots = 0);
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
 // cmax: nfields unsigned of scratch (the first round's largest |count| per field)
 hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cmax, const int* field_list,
