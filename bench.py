@@ -661,8 +661,13 @@ def main():
     fft_gbs = sum(bytes_per_launch[k] * prof[k][1] for k in kinds if prof[k][1]) / (fft_ms * 1e-3) / 1e9
     value = args.steps * F / elapsed
     # whole-frame algorithmic bytes per wavefront-frame: the passes above,
-    # the extraction's c64 read of the field, an f32 OPD and amplitude write
+    # the extraction's c64 read of the field, an f32 OPD and amplitude write;
+    # with the factored geometric phase (lambda_ref / lambda multipliers, the
+    # default path) also the sweep's f32 read of every layer's footprint, its
+    # two f64 planes written and the extraction's read of them
     frame_bytes = (sum(b for _, b in passes) + 8.0 * N * N * F) / F + 8.0 * wl["Nout"] ** 2
+    if ctx.get_tuning("extract_planes") == 1:
+        frame_bytes += (foot * L + 32.0 * N * N * wl["D"]) / F
     c2e = ctx.get_tuning("col2_e") or (32 if 256 <= N <= 2048 else 16)
     ce = ctx.get_tuning("col_e") or 16
     cc = ctx.get_tuning("col_c") or (8 if N == 2048 else 16)
