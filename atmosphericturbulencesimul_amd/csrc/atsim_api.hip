@@ -848,8 +848,8 @@ struct atsim_sim {
     double* s_dev = nullptr;
     int* unwrap_dev = nullptr;
     // phase multipliers that factor as s_wl = geo_a[w] geo_b[l]: the planes
-    // of the geometric phase's layer sums ([D][2][N][N] FP64), made beside
-    // the Fresnel passes
+    // of the geometric phase's layer sums, made beside the Fresnel passes
+    // ([2][D][N][N] FP64: the layers before the last group, then the group)
     bool factored = false;
     double* geo_a = nullptr;  // [W]
     double* geo_b = nullptr;  // [L]
@@ -1381,8 +1381,9 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         // runs beside another's column pass (memory-bound).  All join the
         // context stream before extraction.  The phase planes of the column
         // passes are sampled on set 1's stream beside set 0's first (row)
-        // pass; every other set's first column pass waits for them.  The
-        // extraction's geometric-phase planes follow on that stream.
+        // pass; every other set's first column pass waits for them.  With
+        // factored multipliers the same sweep over the layers makes the
+        // extraction's geometric-phase planes.
         HIPCHK(c, hipEventRecord(s->ev_fork, ms));
         for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
         if (fused && planes) {

@@ -278,7 +278,9 @@ long atsim_sim_gaps(atsim_sim *sim);
  * one factor, 2 extraction, 3 block unwrap, 4 row pass with two x factors
  * and a screen, 5 column pass with two y factors and a screen, 6 sampling of
  * the phase planes that column passes apply, 7 output statistics, 8 the
- * geometric-phase planes of the extraction ("extract_planes"); returns
+ * geometric-phase planes of the extraction ("extract_planes"; with one pass
+ * per gap the same sweep makes the phase planes, and kind 6 records none);
+ * returns
  * the summed duration [ms] and the number of launches recorded since the
  * last enable. */
 int atsim_sim_profile(atsim_sim *sim, int enable);
