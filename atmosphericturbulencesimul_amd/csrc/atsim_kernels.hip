@@ -1343,8 +1343,7 @@ constexpr int EXT_BC = 8;              // block columns a tile row can touch (bl
 template <bool FULL, int MODE>
 __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     static_assert(MODE == 0 || MODE == 2, "sampling or planes");
-    constexpr bool SAMPLE = MODE == 0, OUTPUT = true;
-    constexpr int NW = 8;  // accumulated wavelengths
+    constexpr bool SAMPLE = MODE == 0;
     __shared__ float S[EXT_LC][EXT_SR][EXT_SW];   // staged master rows
     __shared__ double H[EXT_LC][EXT_SR][EXT_W];   // their x-filtered values
     __shared__ double bs[2][8][2][EXT_BC];        // [block row - jb0][w][cos, sin][block column - b0]
@@ -1408,12 +1407,12 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
             ysq[k] = mod_m(p.y0 + jf0 - 1, M);
             xsq[k] = mod_m(p.x0 + xfirst - 1, M);
         }
-        for (int k = threadIdx.x; k < NW * L; k += EXT_T) {
+        for (int k = threadIdx.x; k < 8 * L; k += EXT_T) {
             const int w = k / L, l = k - w * L;
             ss[l][w] = w < Wc ? a.all.s[(w_begin + w) * L + l] : 0.0;
         }
     }
-    if (OUTPUT) {
+    {
         if (threadIdx.x < 8) um[threadIdx.x] = threadIdx.x < Wc ? a.unwrap_mode[w_begin + threadIdx.x] : 0;
         for (int k = threadIdx.x; k < 2 * 8 * 2 * EXT_BC; k += EXT_T) (&bs[0][0][0][0])[k] = 0.0;
         for (int k = threadIdx.x; k < 2 * 8 * EXT_BC; k += EXT_T) (&bm[0][0][0])[k] = 0u;
@@ -1423,7 +1422,7 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
     float2 Ein[8];
 #pragma unroll
     for (int w = 0; w < 8; w++)
-        Ein[w] = (OUTPUT && valid && w < Wc && a.fields)
+        Ein[w] = (valid && w < Wc && a.fields)
                      ? a.fields[((size_t)(d * a.W + w_begin + w) * a.N + jf) * a.N + ifl]
                      : make_float2(1.f, 0.f);
     double PHI[8], PL[8];
@@ -1490,13 +1489,13 @@ __global__ void __launch_bounds__(EXT_T) k_extract(ExtractArgs a) {
                 const double v = sample(q);
                 const double* sl = ss[l0 + q];
 #pragma unroll
-                for (int w = 0; w < NW; w++) PHI[w] = fma(sl[w], v, PHI[w]);
+                for (int w = 0; w < 8; w++) PHI[w] = fma(sl[w], v, PHI[w]);
             }
             for (int q = lo; q < hi; q++) {
                 const double v = sample(q);
                 const double* sl = ss[l0 + q];
 #pragma unroll
-                for (int w = 0; w < NW; w++) PL[w] = fma(sl[w], v, PL[w]);
+                for (int w = 0; w < 8; w++) PL[w] = fma(sl[w], v, PL[w]);
             }
         }
     }
