@@ -865,7 +865,7 @@ struct atsim_sim {
     int* bstart = nullptr;    // [xs + 1] first output row / column of each block, then Nout
     double* bin = nullptr;
     double* ch = nullptr;
-    unsigned* cmax = nullptr;  // [F] largest first-round count of each block-rule field
+    unsigned* cmax = nullptr;  // [F] largest first-round count of each block-rule field (in bsum's allocation)
     int* flist_dev = nullptr;
     // sampled phase planes of the layers whose screens column passes apply
     // (one pass per gap): [nslots][D][N][N] FP64 quarter turns
@@ -984,7 +984,6 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->bstart);
     hipFree(s->bin);
     hipFree(s->ch);
-    hipFree(s->cmax);
     hipFree(s->flist_dev);
     hipFree(s->vq);
     hipFree(s->slot_layer_dev);
@@ -996,9 +995,9 @@ static void sim_free(atsim_sim* s) {
 
 extern "C" {
 
-// doubles of the block-sum allocation: F nb (cos, sin) sums, then F nb
-// unsigned maxima
-static long bsum_doubles(long F, long nb) { return F * nb * 2 + (F * nb + 1) / 2; }
+// doubles of the block-sum allocation, cleared every frame: F nb (cos, sin)
+// sums, then F nb unsigned largest |r| and F unsigned largest |count|
+static long bsum_doubles(long F, long nb) { return F * nb * 2 + (F * nb + F + 1) / 2; }
 
 int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (!c || !d || !out) return ATSIM_EINVAL;
@@ -1118,11 +1117,13 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         SIMALLOC(s->dres, sizeof(float) * F * no);
         // block sums and largest |r|, cleared together every frame
         SIMALLOC(s->bsum, sizeof(double) * bsum_doubles(F, nb));
-        if (s->bsum) s->rmax = reinterpret_cast<unsigned*>(s->bsum + F * nb * 2);
+        if (s->bsum) {
+            s->rmax = reinterpret_cast<unsigned*>(s->bsum + F * nb * 2);
+            s->cmax = s->rmax + F * nb;
+        }
         SIMALLOC(s->bstart, sizeof(int) * (s->xs + 1));
         SIMALLOC(s->bin, sizeof(double) * F * nb);
         SIMALLOC(s->ch, sizeof(double) * F * nb);
-        SIMALLOC(s->cmax, sizeof(unsigned) * F);
         SIMALLOC(s->flist_dev, sizeof(int) * flist.size());
     }
 #undef SIMALLOC
@@ -1827,8 +1828,10 @@ int atsim_block_fixup(atsim_ctx* c, const double* bsum, long xs, double* peaks) 
         hipMalloc(&d_f, sizeof(int)) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "block fix-up: out of device memory");
     const int f0 = 0;
+    const unsigned z0 = 0u;
     if (r == ATSIM_OK && (copy_on_stream(c, d_sum, bsum, sizeof(double) * nb * 2) != hipSuccess ||
-                          copy_on_stream(c, d_f, &f0, sizeof(int)) != hipSuccess))
+                          copy_on_stream(c, d_f, &f0, sizeof(int)) != hipSuccess ||
+                          copy_on_stream(c, d_max, &z0, sizeof(unsigned)) != hipSuccess))
         r = fail(c, ATSIM_EHIP, "block fix-up: upload failed");
     if (r == ATSIM_OK && launch_block_fix(d_sum, d_bin, d_ch, d_max, d_f, 1, (int)xs, c->stream) != hipSuccess)
         r = fail(c, ATSIM_EHIP, "block fix-up: launch failed");

@@ -1773,9 +1773,9 @@ __constant__ int c_pairs[12][6] = {
 // of the pair's base block; each step is a double add rounded to float, so
 // the order shows in the last bits (a count that is 0.5 in exact arithmetic
 // can come out at 0.50000006 and pass plim = 0.5).
-__device__ __forceinline__ float block_count(const double* B, int b, int xs) {
-    const int i2 = b % xs, j2 = b / xs;
-    const double pb = B[b];
+template <class Peak>
+__device__ __forceinline__ float block_count_at(Peak peak, int i2, int j2, int xs) {
+    const double pb = peak(i2, j2);
     float c = 0.f;
     for (int q = 0; q < 12; q++) {
         const int dx1 = c_pairs[q][0], dy1 = c_pairs[q][1], dx2 = c_pairs[q][2], dy2 = c_pairs[q][3];
@@ -1789,7 +1789,7 @@ __device__ __forceinline__ float block_count(const double* B, int b, int xs) {
             const bool as_first = (t == 0) ? !second_first : second_first;
             const int x = i2 - (as_first ? dx1 : dx2), y = j2 - (as_first ? dy1 : dy2);
             if (x < 0 || y < 0 || x >= xs - xm || y >= xs - ym) continue;
-            const double po = as_first ? B[(y + dy2) * xs + x + dx2] : B[(y + dy1) * xs + x + dx1];
+            const double po = as_first ? peak(x + dx2, y + dy2) : peak(x + dx1, y + dy1);
             const double pv1 = as_first ? pb : po, pv2 = as_first ? po : pb;
             if (pv2 > pv1 + M_PI) c = (float)((double)c + (as_first ? cq : -cq));
             if (pv1 > pv2 + M_PI) c = (float)((double)c + (as_first ? -cq : cq));
@@ -1797,19 +1797,52 @@ __device__ __forceinline__ float block_count(const double* B, int b, int xs) {
     }
     return c;
 }
+__device__ __forceinline__ float block_count(const double* B, int b, int xs) {
+    return block_count_at([&](int x, int y) { return B[y * xs + x]; }, b % xs, b / xs, xs);
+}
 
 // The first round of the fix-up, spread over the whole GPU (one round is all
-// that realistic fields take): the peaks atan2 of the block sums, then each
+// that realistic fields take): the peaks atan2 of the block sums, each
 // block's count and, per field, the largest |count| as the bits of a
-// non-negative float (ordered like unsigned integers).
-__global__ void __launch_bounds__(256) k_block_peaks(const double* __restrict__ bsum, double* __restrict__ bin,
-                                                     const int* __restrict__ flist, int xs,
-                                                     unsigned* __restrict__ cmax) {
+// non-negative float (ordered like unsigned integers; cmax is zero before).
+// A workgroup takes a 16 x 16 tile of blocks: it makes the peaks of the tile
+// and of the 2 blocks around it (every relation reaches 2 blocks away) into
+// LDS, writes its own, and counts from LDS.
+constexpr int PC_T = 16, PC_H = PC_T + 4;
+__global__ void __launch_bounds__(PC_T * PC_T)
+k_block_peaks_counts(const double* __restrict__ bsum, double* __restrict__ bin, double* __restrict__ ch,
+                     const int* __restrict__ flist, int xs, unsigned* __restrict__ cmax) {
+    __shared__ double T[PC_H][PC_H];
+    __shared__ float wmax[PC_T * PC_T / 64];
     const int f = flist[blockIdx.y];
-    const int nb = xs * xs;
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cmax[blockIdx.y] = 0u;
-    if (b < nb) bin[(size_t)f * nb + b] = atan2(bsum[((size_t)f * nb + b) * 2 + 1], bsum[((size_t)f * nb + b) * 2]);
+    const size_t nb = (size_t)xs * xs;
+    const int ntx = (xs + PC_T - 1) / PC_T;
+    const int tx0 = (blockIdx.x % ntx) * PC_T, ty0 = (blockIdx.x / ntx) * PC_T;
+    const double* sf = bsum + (size_t)f * nb * 2;
+    for (int e = threadIdx.x; e < PC_H * PC_H; e += PC_T * PC_T) {
+        const int hy = e / PC_H, hx = e - hy * PC_H;
+        const int x = tx0 - 2 + hx, y = ty0 - 2 + hy;
+        T[hy][hx] = (x >= 0 && y >= 0 && x < xs && y < xs)
+                        ? atan2(sf[((size_t)y * xs + x) * 2 + 1], sf[((size_t)y * xs + x) * 2]) : 0.0;
+    }
+    __syncthreads();
+    const int i = threadIdx.x % PC_T, j = threadIdx.x / PC_T;
+    const int i2 = tx0 + i, j2 = ty0 + j;
+    float a = 0.f;
+    if (i2 < xs && j2 < xs) {
+        const size_t b = (size_t)j2 * xs + i2;
+        bin[(size_t)f * nb + b] = T[j + 2][i + 2];
+        const float c = block_count_at([&](int x, int y) { return T[y - ty0 + 2][x - tx0 + 2]; }, i2, j2, xs);
+        reinterpret_cast<float*>(ch)[(size_t)f * nb + b] = c;
+        a = fabsf(c);
+    }
+    // the workgroup's largest |count|, then one atomic per workgroup (atomics
+    // on one field's word serialise in L2)
+    for (int o = 32; o > 0; o >>= 1) a = fmaxf(a, __shfl_down(a, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(&cmax[blockIdx.y], __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 __global__ void __launch_bounds__(256) k_block_counts(const double* __restrict__ bin, double* __restrict__ ch,
@@ -1835,8 +1868,8 @@ __global__ void __launch_bounds__(256) k_block_counts(const double* __restrict__
 }
 
 // The iterative fix-up from the first round's peaks, counts and largest
-// count (k_block_peaks, k_block_counts); further rounds on one workgroup per
-// field, the block grid in LDS.
+// count (k_block_peaks_counts); further rounds on one workgroup per field,
+// the block grid in LDS.
 __global__ void __launch_bounds__(1024) k_block_fix(double* __restrict__ bin, double* __restrict__ ch,
                                                     const int* __restrict__ flist, int xs, int use_lds,
                                                     const unsigned* __restrict__ cmax) {
@@ -1908,9 +1941,8 @@ hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cma
                             int nfields, int xs, hipStream_t st) {
     if (nfields <= 0 || xs <= 0) return hipSuccess;
     const int nb = xs * xs;
-    dim3 grid((nb + 255) / 256, nfields);
-    k_block_peaks<<<grid, 256, 0, st>>>(bsum, bin, field_list, xs, cmax);
-    k_block_counts<<<grid, 256, 0, st>>>(bin, ch, field_list, xs, cmax);
+    const int nt = (xs + PC_T - 1) / PC_T;
+    k_block_peaks_counts<<<dim3(nt * nt, nfields), PC_T * PC_T, 0, st>>>(bsum, bin, ch, field_list, xs, cmax);
     const size_t need = (size_t)nb * (sizeof(double) + sizeof(float));
     const int use_lds = need <= 150 * 1024;
     static bool attr = false;

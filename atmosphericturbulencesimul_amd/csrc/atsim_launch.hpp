@@ -90,7 +90,8 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st, double* vq = nullptr,
                                    const int* slot_layer = nullptr, int nslots = 0);
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
-// cmax: nfields unsigned of scratch (the first round's largest |count| per field)
+// cmax: nfields unsigned, zero on entry (the first round's largest |count|
+// per field)
 hipError_t launch_block_fix(double* bsum, double* bin, double* ch, unsigned* cmax, const int* field_list,
                             int nfields, int xs, hipStream_t st);
 // first-round counts alone (float, per field nb = xs^2), with cmax as above
