@@ -1364,6 +1364,10 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         a.vl = s->vgeo + (size_t)D * s->N * s->N;
     }
     bool geometry_done = false;
+    // the block sums are cleared by the planes sweep when there is one
+    const bool clear_sums = s->nblockfields > 0 && s->xs > 0;
+    double* const zp = (planes && clear_sums) ? s->bsum : nullptr;
+    const long zn = clear_sums ? bsum_doubles(F, (long)s->xs * s->xs) : 0;
     // the sparse final update of the block rule makes the remainders again
     // from the final fields and the planes (the extraction then writes no
     // map); without propagation there are no fields (the field is 1)
@@ -1390,7 +1394,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
         if (fused && planes) {
             // the phase planes and the geometric-phase planes in one sweep
             // over the layers (joins the context stream with set 1's chain)
-            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], s->vq, s->slot_layer_dev, s->nslots));
+            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], s->vq, s->slot_layer_dev, s->nslots, zp, zn));
             HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
             geometry_done = true;
         } else {
@@ -1399,7 +1403,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
                 HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
             }
             if (planes) {
-                HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0]));
+                HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], nullptr, nullptr, 0, zp, zn));
                 geometry_done = true;
             }
         }
@@ -1424,7 +1428,7 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     } else if (s->propagate) {
         if (fused && planes) {
             long pg = prof_begin(s, PROF_GEOMETRY);
-            HIPCHK(c, launch_extract_geometry(a, full, ms, s->vq, s->slot_layer_dev, s->nslots));
+            HIPCHK(c, launch_extract_geometry(a, full, ms, s->vq, s->slot_layer_dev, s->nslots, zp, zn));
             prof_end(s, pg);
             geometry_done = true;
         } else if (fused) {
@@ -1440,11 +1444,10 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     }
     if (planes && !geometry_done) {
         long pg = prof_begin(s, PROF_GEOMETRY);
-        HIPCHK(c, launch_extract_geometry(a, full, ms));
+        HIPCHK(c, launch_extract_geometry(a, full, ms, nullptr, nullptr, 0, zp, zn));
         prof_end(s, pg);
     }
-    if (s->nblockfields > 0 && s->xs > 0)
-        HIPCHK(c, launch_zero_f64(s->bsum, bsum_doubles(F, (long)s->xs * s->xs), ms));
+    if (clear_sums && !planes) HIPCHK(c, launch_zero_f64(s->bsum, zn, ms));
     long pe = prof_begin(s, PROF_EXTRACT);
     HIPCHK(c, launch_extract(a, full, ms));
     prof_end(s, pe);

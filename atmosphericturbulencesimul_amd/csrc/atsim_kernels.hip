@@ -1638,8 +1638,13 @@ __global__ void __launch_bounds__(GEO_W)
 k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __restrict__ masters,
              const LayerSample* __restrict__ ls, const double* __restrict__ geo_b, int N, int M, int L, int lf, int lc,
              int r_first, int nrows, int c_first, int ncols, double* __restrict__ vq,
-             const int* __restrict__ slot_layer, int nslots) {
+             const int* __restrict__ slot_layer, int nslots, double* __restrict__ zero_p, long zero_n) {
     __shared__ float S[GEO_SR][GEO_W + 4];
+    // the extraction's block sums of this frame start from zero (cleared here
+    // rather than by a launch of their own)
+    for (long e = ((long)blockIdx.y * gridDim.x + blockIdx.x) * GEO_W + threadIdx.x; e < zero_n;
+         e += (long)gridDim.x * gridDim.y * GEO_W)
+        zero_p[e] = 0.0;
     const int ntx = (ncols + GEO_W - 1) / GEO_W;
     int tx, ty;
     if ((ntx & 7) == 0) {
@@ -1728,7 +1733,7 @@ k_geo_planes(double* __restrict__ vr, double* __restrict__ vl, const float* __re
 }
 
 hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st, double* vq,
-                                   const int* slot_layer, int nslots) {
+                                   const int* slot_layer, int nslots, double* zero_p, long zero_n) {
     if (a.vr == nullptr || a.vl == nullptr || a.geo_b == nullptr) return hipErrorInvalidValue;
     if (vq != nullptr && (slot_layer == nullptr || nslots <= 0)) return hipErrorInvalidValue;
     // the column passes' phase planes cover the whole grid
@@ -1739,7 +1744,8 @@ hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t 
     if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
     dim3 grid((unsigned)tiles, a.D), block(GEO_W);
     k_geo_planes<<<grid, block, 0, st>>>(a.vr, a.vl, a.all.masters, a.all.ls, a.geo_b, a.N, (int)a.all.M, a.all.L,
-                                         a.last_first, a.last_count, first, n, first, n, vq, slot_layer, nslots);
+                                         a.last_first, a.last_count, first, n, first, n, vq, slot_layer, nslots,
+                                         zero_p, zero_p ? zero_n : 0);
     return hipGetLastError();
 }
 

@@ -86,9 +86,10 @@ hipError_t launch_tf_init(float2* C, long size, double coeff, double fpmaskrad, 
 
 // the planes a.vr, a.vl of the factored geometric phase, and with vq the
 // column passes' phase planes of the nslots layers slot_layer as
-// launch_layer_phase makes them
+// launch_layer_phase makes them; zero_n doubles at zero_p are cleared
 hipError_t launch_extract_geometry(const ExtractArgs& a, bool full, hipStream_t st, double* vq = nullptr,
-                                   const int* slot_layer = nullptr, int nslots = 0);
+                                   const int* slot_layer = nullptr, int nslots = 0, double* zero_p = nullptr,
+                                   long zero_n = 0);
 hipError_t launch_extract(const ExtractArgs& a, bool full, hipStream_t st);
 // cmax: nfields unsigned, zero on entry (the first round's largest |count|
 // per field)
