@@ -42,7 +42,14 @@ struct atsim_ctx {
     // 1: the block rule's final update visits only the blocks whose largest
     // |r| can give a non-zero count; 2: every pixel of every block
     int block_apply = 1;
+    // 1: frames of a simulation overlap: frame k + 1's Fresnel passes run on
+    // the simulation's own stream beside frame k's extraction and block
+    // fix-up, which stay on the context stream (two buffer sets alternate)
+    int pipeline = 0;
     unsigned tuning_gen = 0;  // bumped by every accepted atsim_set_tuning: captured frame graphs are stale
+    // bumped by every call that enqueues device writes on the context stream:
+    // a pipelined frame's passes then wait for that stream first
+    unsigned long work_gen = 0;
     // asynchronous device-to-host copies (default priority, like the main stream)
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ready[ATSIM_COPY_SLOTS] = {};
@@ -275,6 +282,7 @@ int atsim_free(atsim_ctx* c, void* ptr) {
 
 int atsim_copy(atsim_ctx* c, void* dst, const void* src, size_t nbytes) {
     if (!c || !dst || !src) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, copy_on_stream(c, dst, src, nbytes));
     return ATSIM_OK;
 }
@@ -359,6 +367,9 @@ int atsim_set_tuning(atsim_ctx* c, const char* key, int value) {
     } else if (!strcmp(key, "extract_planes")) {
         if (value < 0 || value > 2) return fail(c, ATSIM_EINVAL, "extract_planes must be 0, 1 or 2");
         c->extract_planes = value == 0 ? 1 : value;
+    } else if (!strcmp(key, "pipeline")) {
+        if (value != 0 && value != 1) return fail(c, ATSIM_EINVAL, "pipeline must be 0 or 1");
+        c->pipeline = value;
     } else {
         return fail(c, ATSIM_EINVAL, "unknown tuning key '%s'", key);
     }
@@ -377,6 +388,7 @@ int atsim_get_tuning(atsim_ctx* c, const char* key) {
     if (!strcmp(key, "graph")) return c->graph;
     if (!strcmp(key, "extract_planes")) return c->extract_planes;
     if (!strcmp(key, "block_apply")) return c->block_apply;
+    if (!strcmp(key, "pipeline")) return c->pipeline;
     return ATSIM_EINVAL;
 }
 
@@ -390,6 +402,7 @@ int atsim_device_ok(atsim_ctx* c) {
 // ------------------------------------------------------------------ FFT
 int atsim_fft2d(atsim_ctx* c, void* d_data, long n, long nbatch, int sign) {
     if (!c || !d_data) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fft2d: size %ld is not a power of two in [16, 8192]", n);
     if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fft2d: nbatch %ld out of range", nbatch);
     if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "fft2d: sign must be -1 or +1");
@@ -403,6 +416,7 @@ int atsim_fft2d(atsim_ctx* c, void* d_data, long n, long nbatch, int sign) {
 
 int atsim_fft2d_variant(atsim_ctx* c, void* d_data, long n, long nbatch, int sign, int variant) {
     if (!c || !d_data) return ATSIM_EINVAL;
+    c->work_gen++;
     if (variant == 0) return atsim_fft2d(c, d_data, n, nbatch, sign);
     if (variant != 1) return fail(c, ATSIM_EINVAL, "fft2d_variant: unknown variant %d", variant);
     if (!fft_mfma_supported(n))
@@ -427,6 +441,7 @@ int atsim_fft2d_variant(atsim_ctx* c, void* d_data, long n, long nbatch, int sig
 
 int atsim_row_fft_variant(atsim_ctx* c, void* d_data, long n, long nrows, int sign, int variant) {
     if (!c || !d_data) return ATSIM_EINVAL;
+    c->work_gen++;
     if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "row_fft_variant: sign must be -1 or +1");
     if (nrows <= 0 || nrows % n) return fail(c, ATSIM_EINVAL, "row_fft_variant: nrows must be a multiple of n");
     float2* tw;
@@ -445,6 +460,7 @@ int atsim_row_fft_variant(atsim_ctx* c, void* d_data, long n, long nrows, int si
 
 int atsim_permut(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch) {
     if (!c || !d_data) return ATSIM_EINVAL;
+    c->work_gen++;
     if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "permut: sizes must be > 0");
     if (!(n0 & 1) && !(n1 & 1)) {
         HIPCHK(c, launch_permut((float2*)d_data, n0, n1, nbatch, c->stream));
@@ -544,6 +560,7 @@ static int axis_transform(atsim_ctx* c, float2* data, long L, long nlines, long 
 
 int atsim_fft2d_nm(atsim_ctx* c, void* d_data, long n0, long n1, long nbatch, int sign) {
     if (!c || !d_data) return ATSIM_EINVAL;
+    c->work_gen++;
     if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "fft2d_nm: sizes must be > 0");
     if (sign != -1 && sign != 1) return fail(c, ATSIM_EINVAL, "fft2d_nm: sign must be -1 or +1");
     if (n0 == n1 && supported_n(n0) && nbatch <= 65535) return atsim_fft2d(c, d_data, n0, nbatch, sign);
@@ -568,6 +585,7 @@ static int fresnel_run(atsim_ctx* c, float2* d, long n, long nbatch, const float
 int atsim_fresnel(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbatch, double dx, double z,
                   double lambda) {
     if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel: size %ld is not a power of two in [16, 8192]", n);
     if (nbatch <= 0 || nbatch > 65535) return fail(c, ATSIM_EINVAL, "fresnel: nbatch %ld out of range", nbatch);
     if (!(dx > 0.0)) return fail(c, ATSIM_EINVAL, "fresnel: pupil scale must be > 0");
@@ -587,6 +605,7 @@ int atsim_fresnel(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbat
 int atsim_fresnel_nm(atsim_ctx* c, const void* d_in, void* d_out, long n0, long n1, long nbatch, double dx,
                      double z, double lambda) {
     if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    c->work_gen++;
     if (n0 <= 0 || n1 <= 0 || nbatch <= 0) return fail(c, ATSIM_EINVAL, "fresnel_nm: sizes must be > 0");
     if (!(dx > 0.0)) return fail(c, ATSIM_EINVAL, "fresnel_nm: pupil scale must be > 0");
     if (n0 == n1 && supported_n(n0) && nbatch <= 65535) return atsim_fresnel(c, d_in, d_out, n0, nbatch, dx, z, lambda);
@@ -619,6 +638,7 @@ int atsim_fresnel_nm(atsim_ctx* c, const void* d_in, void* d_out, long n0, long 
 int atsim_fresnel_noalloc(atsim_ctx* c, const void* d_in, void* d_out, long n, long nbatch, double dx, double z,
                           double lambda) {
     if (!c || !d_in || !d_out) return ATSIM_EINVAL;
+    c->work_gen++;
     if (n > ATSIM_NOALLOC_MAX_N)
         return fail(c, ATSIM_EUNSUPPORTED, "fresnel_noalloc: size %ld exceeds the maximum %d", n, ATSIM_NOALLOC_MAX_N);
     if (!supported_n(n))
@@ -644,6 +664,7 @@ int atsim_fresnel_noalloc(atsim_ctx* c, const void* d_in, void* d_out, long n, l
 
 int atsim_fresnel_tf_init(atsim_ctx* c, void* d_C, long n, double dx, double z, double lambda, double fpmaskrad) {
     if (!c || !d_C) return ATSIM_EINVAL;
+    c->work_gen++;
     if (n <= 0 || (n & 1)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_tf_init: even size required (got %ld)", n);
     const double coeff = M_PI * z * lambda / (dx * n) / (dx * n);
     HIPCHK(c, launch_tf_init((float2*)d_C, n, coeff, fpmaskrad, c->stream));
@@ -652,6 +673,7 @@ int atsim_fresnel_tf_init(atsim_ctx* c, void* d_C, long n, double dx, double z, 
 
 int atsim_fresnel_tf_apply(atsim_ctx* c, const void* d_in, void* d_out, const void* d_C, long n, long nbatch) {
     if (!c || !d_in || !d_out || !d_C) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_tf_apply: size %ld unsupported", n);
     if (d_in != d_out)
         HIPCHK(c, hipMemcpyAsync(d_out, d_in, sizeof(float2) * n * n * nbatch, hipMemcpyDeviceToDevice, c->stream));
@@ -663,6 +685,7 @@ int atsim_fresnel_tf_apply(atsim_ctx* c, const void* d_in, void* d_out, const vo
 int atsim_fresnel_cube(atsim_ctx* c, const void* d_in, float* d_amp, float* d_pha, long n, double dx,
                        double zstart, double zend, long nz, double lambda) {
     if (!c || !d_in || !d_amp || !d_pha) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!supported_n(n)) return fail(c, ATSIM_EUNSUPPORTED, "fresnel_cube: size %ld unsupported", n);
     if (nz <= 0) return fail(c, ATSIM_EINVAL, "fresnel_cube: nz must be > 0");
     float2* tmp = nullptr;
@@ -684,6 +707,7 @@ int atsim_fresnel_cube(atsim_ctx* c, const void* d_in, float* d_amp, float* d_ph
 // ------------------------------------------------------------------ noise
 int atsim_noise_c64(atsim_ctx* c, void* d_out, long n, uint64_t seed, uint32_t stream) {
     if (!c || !d_out || n <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_noise((float2*)d_out, n, seed, stream, c->stream));
     return ATSIM_OK;
 }
@@ -702,6 +726,7 @@ static int reduce_partials(atsim_ctx* c, double* d_part, int nblocks, int nv, st
 int atsim_make_master(atsim_ctx* c, const void* d_noise, long size, double outerscale_px, double innerscale_px,
                       double rlim, float* d_out1, float* d_out2, double* C1, double* C2) {
     if (!c || !d_noise || !d_out1) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!supported_n(size)) return fail(c, ATSIM_EUNSUPPORTED, "make_master: size %ld unsupported", size);
     if (!(outerscale_px > 0) || !(innerscale_px > 0)) return fail(c, ATSIM_EINVAL, "make_master: scales must be > 0");
     const double f0 = 1.0 * size / outerscale_px;                        // AT.c:679
@@ -778,6 +803,7 @@ int atsim_r0_measure(atsim_ctx* c, const float* d_screen, long size, long dpix, 
 
 int atsim_scale(atsim_ctx* c, float* d_screen, long n, double scale) {
     if (!c || !d_screen || n <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_scale(d_screen, n, scale, c->stream));
     return ATSIM_OK;
 }
@@ -837,6 +863,8 @@ struct atsim_sim {
     std::vector<Group> groups;  // top -> bottom
     bool propagate = false;     // Fresnel passes run this frame sequence
     float* masters = nullptr;
+    // the current frame's buffer set (use_buffers): the fields, the layer
+    // positions, the geometric phase's planes and the block sums
     float2* fields = nullptr;
     float2* tf = nullptr;       // [gap][W][N]
     float2* tw = nullptr;
@@ -899,7 +927,27 @@ struct atsim_sim {
         float* amp = nullptr;
         void* field = nullptr;
         long used = -1;
-    } gslot[2];
+        // pipelined frames: exec holds the Fresnel passes, exec2 the
+        // extraction and block fix-up; the buffer set they were captured on
+        hipGraph_t graph2 = nullptr;
+        hipGraphExec_t exec2 = nullptr;
+        int parity = 0;
+        bool split = false;
+    } gslot[4];
+    // pipelined frames (tuning key "pipeline"): buffer sets 0 and 1 alternate
+    // between frames; set 1 is allocated at the first pipelined frame
+    float2* fields_buf[2] = {};
+    LayerSample* ls_buf[2] = {};
+    double* vgeo_buf[2] = {};
+    double* bsum_buf[2] = {};
+    int parity = 0;
+    hipStream_t hs = nullptr;  // the passes of pipelined frames
+    hipEvent_t ev_head[2] = {}, ev_tail[2] = {}, ev_sync = nullptr;
+    bool tail_rec[2] = {false, false};
+    // the next pipelined frame's passes wait for the context stream: a frame
+    // ran there without pipelining, or the simulation's inputs changed
+    bool head_sync = true;
+    unsigned long seen_gen = 0;
 };
 
 enum {
@@ -944,6 +992,8 @@ static void prof_end(atsim_sim* s, long i0) {
 static void drop_graph_slot(atsim_sim::GraphSlot& g) {
     if (g.exec) hipGraphExecDestroy(g.exec);
     if (g.graph) hipGraphDestroy(g.graph);
+    if (g.exec2) hipGraphExecDestroy(g.exec2);
+    if (g.graph2) hipGraphDestroy(g.graph2);
     g = atsim_sim::GraphSlot{};
 }
 
@@ -954,13 +1004,22 @@ static void sim_drop_graph(atsim_sim* s) {
 static void sim_free(atsim_sim* s) {
     if (!s) return;
     if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    if (s->hs) hipStreamSynchronize(s->hs);
     sim_drop_graph(s);
     if (s->own) hipStreamDestroy(s->own);
     if (s->own_done) hipEventDestroy(s->own_done);
     if (!s->masters_borrowed) hipFree(s->masters);
-    hipFree(s->fields);
+    for (int p = 0; p < 2; p++) {
+        hipFree(s->fields_buf[p]);
+        hipFree(s->ls_buf[p]);
+        hipFree(s->vgeo_buf[p]);
+        hipFree(s->bsum_buf[p]);
+        if (s->ev_head[p]) hipEventDestroy(s->ev_head[p]);
+        if (s->ev_tail[p]) hipEventDestroy(s->ev_tail[p]);
+    }
+    if (s->ev_sync) hipEventDestroy(s->ev_sync);
+    if (s->hs) hipStreamDestroy(s->hs);
     hipFree(s->tf);
-    hipFree(s->ls_dev);
     if (s->ls_host) hipHostFree(s->ls_host);
     if (s->ls_ev) {
         for (int i = 0; i < atsim_sim::RING; i++) hipEventDestroy(s->ls_ev[i]);
@@ -969,7 +1028,6 @@ static void sim_free(atsim_sim* s) {
     hipFree(s->s_dev);
     hipFree(s->geo_a);
     hipFree(s->geo_b);
-    hipFree(s->vgeo);
     hipFree(s->unwrap_dev);
     hipFree(s->dres);
     for (int k = 0; k < atsim_sim::MAX_SETS - 1; k++) {
@@ -980,7 +1038,6 @@ static void sim_free(atsim_sim* s) {
     if (s->ev_fork) hipEventDestroy(s->ev_fork);
     if (s->ev_phase) hipEventDestroy(s->ev_phase);
     hipFree(s->blk);
-    hipFree(s->bsum);
     hipFree(s->bstart);
     hipFree(s->bin);
     hipFree(s->ch);
@@ -998,6 +1055,58 @@ extern "C" {
 // doubles of the block-sum allocation, cleared every frame: F nb (cos, sin)
 // sums, then F nb unsigned largest |r| and F unsigned largest |count|
 static long bsum_doubles(long F, long nb) { return F * nb * 2 + (F * nb + F + 1) / 2; }
+
+// Make buffer set p the current frame's: fields, layer positions, geometric
+// phase planes and block sums (with each block's largest |r| and each
+// field's largest count in the same allocation).
+static void use_buffers(atsim_sim* s, int p) {
+    s->parity = p;
+    s->fields = s->fields_buf[p];
+    s->ls_dev = s->ls_buf[p];
+    s->vgeo = s->vgeo_buf[p];
+    s->bsum = s->bsum_buf[p];
+    s->rmax = nullptr;
+    s->cmax = nullptr;
+    if (s->bsum) {
+        const long F = s->D * s->W, nb = s->xs > 0 ? (long)s->xs * s->xs : 1;
+        s->rmax = reinterpret_cast<unsigned*>(s->bsum + F * nb * 2);
+        s->cmax = s->rmax + F * nb;
+    }
+}
+
+// Buffer set 1, the passes' stream and the events of pipelined frames,
+// made at the first such frame.
+static int pipeline_init(atsim_sim* s) {
+    if (s->hs) return ATSIM_OK;
+    atsim_ctx* c = s->ctx;
+    const long F = s->D * s->W;
+    const size_t nn = (size_t)s->N * s->N;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (*p) return ATSIM_OK;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            return fail(c, ATSIM_ENOMEM, "pipeline: cannot allocate %zu bytes for a second buffer set", bytes);
+        }
+        return ATSIM_OK;
+    };
+    if (s->fields_buf[0])
+        if (int r = alloc((void**)&s->fields_buf[1], sizeof(float2) * F * nn)) return r;
+    if (int r = alloc((void**)&s->ls_buf[1], sizeof(LayerSample) * s->D * s->L)) return r;
+    if (s->vgeo_buf[0])
+        if (int r = alloc((void**)&s->vgeo_buf[1], sizeof(double) * 2 * s->D * nn)) return r;
+    if (s->bsum_buf[0]) {
+        const long nb = s->xs > 0 ? (long)s->xs * s->xs : 1;
+        if (int r = alloc((void**)&s->bsum_buf[1], sizeof(double) * bsum_doubles(F, nb))) return r;
+    }
+    for (int p = 0; p < 2; p++) {
+        if (!s->ev_head[p]) HIPCHK(c, hipEventCreateWithFlags(&s->ev_head[p], hipEventDisableTiming));
+        if (!s->ev_tail[p]) HIPCHK(c, hipEventCreateWithFlags(&s->ev_tail[p], hipEventDisableTiming));
+    }
+    if (!s->ev_sync) HIPCHK(c, hipEventCreateWithFlags(&s->ev_sync, hipEventDisableTiming));
+    HIPCHK(c, hipStreamCreateWithFlags(&s->hs, hipStreamNonBlocking));
+    s->head_sync = true;
+    return ATSIM_OK;
+}
 
 int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (!c || !d || !out) return ATSIM_EINVAL;
@@ -1056,10 +1165,10 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     } while (0)
     SIMALLOC(s->masters, sizeof(float) * s->L * s->M * s->M);
     if (s->propagate) {
-        SIMALLOC(s->fields, sizeof(float2) * F * nn);
+        SIMALLOC(s->fields_buf[0], sizeof(float2) * F * nn);
         SIMALLOC(s->tf, sizeof(float2) * G * s->W * s->N);
     }
-    SIMALLOC(s->ls_dev, sizeof(LayerSample) * s->D * s->L);
+    SIMALLOC(s->ls_buf[0], sizeof(LayerSample) * s->D * s->L);
     SIMALLOC(s->s_dev, sizeof(double) * s->W * s->L);
     SIMALLOC(s->unwrap_dev, sizeof(int) * s->W);
     SIMALLOC(s->blk, sizeof(int) * (s->Nout > 0 ? s->Nout : 1));
@@ -1077,7 +1186,7 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
     if (s->factored) {
         SIMALLOC(s->geo_a, sizeof(double) * s->W);
         SIMALLOC(s->geo_b, sizeof(double) * s->L);
-        SIMALLOC(s->vgeo, sizeof(double) * 2 * s->D * nn);
+        SIMALLOC(s->vgeo_buf[0], sizeof(double) * 2 * s->D * nn);
     }
     if (r == ATSIM_OK && hipHostMalloc((void**)&s->ls_host, sizeof(LayerSample) * s->D * s->L * atsim_sim::RING) != hipSuccess)
         r = fail(c, ATSIM_ENOMEM, "sim: pinned allocation failed");
@@ -1116,17 +1225,14 @@ int atsim_sim_create(atsim_ctx* c, const atsim_sim_desc* d, atsim_sim** out) {
         const size_t nb = (size_t)(s->xs > 0 ? s->xs * s->xs : 1);
         SIMALLOC(s->dres, sizeof(float) * F * no);
         // block sums and largest |r|, cleared together every frame
-        SIMALLOC(s->bsum, sizeof(double) * bsum_doubles(F, nb));
-        if (s->bsum) {
-            s->rmax = reinterpret_cast<unsigned*>(s->bsum + F * nb * 2);
-            s->cmax = s->rmax + F * nb;
-        }
+        SIMALLOC(s->bsum_buf[0], sizeof(double) * bsum_doubles(F, nb));
         SIMALLOC(s->bstart, sizeof(int) * (s->xs + 1));
         SIMALLOC(s->bin, sizeof(double) * F * nb);
         SIMALLOC(s->ch, sizeof(double) * F * nb);
         SIMALLOC(s->flist_dev, sizeof(int) * flist.size());
     }
 #undef SIMALLOC
+    use_buffers(s, 0);
     if (r == ATSIM_OK && hipMemsetAsync(s->masters, 0, sizeof(float) * s->L * s->M * s->M, c->stream) != hipSuccess)
         r = fail(c, ATSIM_EHIP, "sim: memset failed");
     if (r == ATSIM_OK) r = get_twiddles(c, s->N, &s->tw);
@@ -1172,11 +1278,14 @@ void atsim_sim_destroy(atsim_sim* s) { sim_free(s); }
 
 float* atsim_sim_master(atsim_sim* s, long layer) {
     if (!s || layer < 0 || layer >= s->L) return nullptr;
+    s->head_sync = true;
     return s->masters + (size_t)layer * s->M * s->M;
 }
 
 int atsim_sim_set_master(atsim_sim* s, long layer, const float* src) {
     if (!s || !src || layer < 0 || layer >= s->L) return ATSIM_EINVAL;
+    s->head_sync = true;
+    s->ctx->work_gen++;
     atsim_ctx* c = s->ctx;
     HIPCHK(c, copy_on_stream(c, s->masters + (size_t)layer * s->M * s->M, src, sizeof(float) * s->M * s->M));
     return ATSIM_OK;
@@ -1217,7 +1326,7 @@ static void tap_weights(double f, int bicubic, double w[4]) {
 
 // one time step of the layer positions (AT.c:2706-2755: cumulative FP64,
 // wrapped into [0, M]) and the upload of their integer bases and tap weights
-static int advance_positions(atsim_sim* s) {
+static int advance_positions(atsim_sim* s, hipStream_t st) {
     atsim_ctx* c = s->ctx;
     const long L = s->L, D = s->D, M = s->M;
     LayerSample* slot = s->ls_host + (size_t)s->ls_slot * D * L;
@@ -1240,15 +1349,16 @@ static int advance_positions(atsim_sim* s) {
             tap_weights(x - xi, s->bicubic, p.wx);
             tap_weights(y - yi, s->bicubic, p.wy);
         }
-    HIPCHK(c, hipMemcpyAsync(s->ls_dev, slot, sizeof(LayerSample) * D * L, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipEventRecord(s->ls_ev[s->ls_slot], c->stream));
+    HIPCHK(c, hipMemcpyAsync(s->ls_dev, slot, sizeof(LayerSample) * D * L, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipEventRecord(s->ls_ev[s->ls_slot], st));
     s->ls_slot = (s->ls_slot + 1) % atsim_sim::RING;
     return ATSIM_OK;
 }
 
 int atsim_sim_advance(atsim_sim* s) {
     if (!s) return ATSIM_EINVAL;
-    return advance_positions(s);
+    s->head_sync = true;
+    return advance_positions(s, s->ctx->stream);
 }
 
 // Fresnel passes of a frame.  The x and y factors of a gap commute.  With
@@ -1309,7 +1419,12 @@ static hipError_t launch_step(atsim_sim* s, bool fused, long step, ScreenArgs sa
 
 // the kernels of one frame (positions already uploaded), in stream order on
 // ms (the context stream unless a multi-simulation frame gives another)
-static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field, hipStream_t ms = nullptr) {
+// Parts of a frame: the Fresnel passes with the planes they read, then the
+// extraction, block fix-up and statistics.
+enum { FRAME_PASSES = 1, FRAME_EXTRACT = 2, FRAME_ALL = 3 };
+
+static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field, hipStream_t ms = nullptr,
+                         int part = FRAME_ALL) {
     atsim_ctx* c = s->ctx;
     if (!ms) ms = c->stream;
     const long L = s->L, D = s->D, W = s->W, M = s->M;
@@ -1379,75 +1494,78 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     auto phase = [&](hipStream_t st) {
         return launch_layer_phase(s->vq, s->masters, s->ls_dev, s->slot_layer_dev, s->nslots, s->N, M, L, D, st);
     };
-    if (s->propagate && nsets >= 2 && !s->prof) {
-        // nsets wavelength sets, each its own chain of passes: set 0 on the
-        // context stream, set k on xs_stream[k - 1], started after set
-        // k - 1's first pass, so that one set's row pass (arithmetic-bound)
-        // runs beside another's column pass (memory-bound).  All join the
-        // context stream before extraction.  The phase planes of the column
-        // passes are sampled on set 1's stream beside set 0's first (row)
-        // pass; every other set's first column pass waits for them.  With
-        // factored multipliers the same sweep over the layers makes the
-        // extraction's geometric-phase planes.
-        HIPCHK(c, hipEventRecord(s->ev_fork, ms));
-        for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
-        if (fused && planes) {
-            // the phase planes and the geometric-phase planes in one sweep
-            // over the layers (joins the context stream with set 1's chain)
-            HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], s->vq, s->slot_layer_dev, s->nslots, zp, zn));
-            HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
-            geometry_done = true;
-        } else {
-            if (fused) {
-                HIPCHK(c, phase(s->xs_stream[0]));
+    if (part & FRAME_PASSES) {
+        if (s->propagate && nsets >= 2 && !s->prof) {
+            // nsets wavelength sets, each its own chain of passes: set 0 on the
+            // context stream, set k on xs_stream[k - 1], started after set
+            // k - 1's first pass, so that one set's row pass (arithmetic-bound)
+            // runs beside another's column pass (memory-bound).  All join the
+            // context stream before extraction.  The phase planes of the column
+            // passes are sampled on set 1's stream beside set 0's first (row)
+            // pass; every other set's first column pass waits for them.  With
+            // factored multipliers the same sweep over the layers makes the
+            // extraction's geometric-phase planes.
+            HIPCHK(c, hipEventRecord(s->ev_fork, ms));
+            for (int k = 1; k < nsets; k++) HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k - 1], s->ev_fork, 0));
+            if (fused && planes) {
+                // the phase planes and the geometric-phase planes in one sweep
+                // over the layers (joins the context stream with set 1's chain)
+                HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], s->vq, s->slot_layer_dev, s->nslots, zp, zn));
                 HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
-            }
-            if (planes) {
-                HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], nullptr, nullptr, 0, zp, zn));
                 geometry_done = true;
-            }
-        }
-        for (long step = 0; step < nsteps; step++) {
-            for (int k = 0; k < nsets; k++) {
-                hipStream_t st = k ? s->xs_stream[k - 1] : ms;
-                sa.wlo = (int)(W * k / nsets);
-                sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
-                if (fused && step == 1 && k != 1) HIPCHK(c, hipStreamWaitEvent(st, s->ev_phase, 0));
-                HIPCHK(c, launch_step(s, fused, step, sa, st));
-                if (step == 0 && k + 1 < nsets) {
-                    HIPCHK(c, hipEventRecord(s->ev_first[k], st));
-                    HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k], s->ev_first[k], 0));
+            } else {
+                if (fused) {
+                    HIPCHK(c, phase(s->xs_stream[0]));
+                    HIPCHK(c, hipEventRecord(s->ev_phase, s->xs_stream[0]));
+                }
+                if (planes) {
+                    HIPCHK(c, launch_extract_geometry(a, full, s->xs_stream[0], nullptr, nullptr, 0, zp, zn));
+                    geometry_done = true;
                 }
             }
+            for (long step = 0; step < nsteps; step++) {
+                for (int k = 0; k < nsets; k++) {
+                    hipStream_t st = k ? s->xs_stream[k - 1] : ms;
+                    sa.wlo = (int)(W * k / nsets);
+                    sa.wcnt = (int)(W * (k + 1) / nsets) - sa.wlo;
+                    if (fused && step == 1 && k != 1) HIPCHK(c, hipStreamWaitEvent(st, s->ev_phase, 0));
+                    HIPCHK(c, launch_step(s, fused, step, sa, st));
+                    if (step == 0 && k + 1 < nsets) {
+                        HIPCHK(c, hipEventRecord(s->ev_first[k], st));
+                        HIPCHK(c, hipStreamWaitEvent(s->xs_stream[k], s->ev_first[k], 0));
+                    }
+                }
+            }
+            sa.wlo = sa.wcnt = 0;
+            for (int k = 1; k < nsets; k++) {
+                HIPCHK(c, hipEventRecord(s->ev_join[k - 1], s->xs_stream[k - 1]));
+                HIPCHK(c, hipStreamWaitEvent(ms, s->ev_join[k - 1], 0));
+            }
+        } else if (s->propagate) {
+            if (fused && planes) {
+                long pg = prof_begin(s, PROF_GEOMETRY);
+                HIPCHK(c, launch_extract_geometry(a, full, ms, s->vq, s->slot_layer_dev, s->nslots, zp, zn));
+                prof_end(s, pg);
+                geometry_done = true;
+            } else if (fused) {
+                long pp = prof_begin(s, PROF_PHASE);
+                HIPCHK(c, phase(ms));
+                prof_end(s, pp);
+            }
+            for (long step = 0; step < nsteps; step++) {
+                long pr = prof_begin(s, step_kind(fused, step, G));
+                HIPCHK(c, launch_step(s, fused, step, sa, ms));
+                prof_end(s, pr);
+            }
         }
-        sa.wlo = sa.wcnt = 0;
-        for (int k = 1; k < nsets; k++) {
-            HIPCHK(c, hipEventRecord(s->ev_join[k - 1], s->xs_stream[k - 1]));
-            HIPCHK(c, hipStreamWaitEvent(ms, s->ev_join[k - 1], 0));
-        }
-    } else if (s->propagate) {
-        if (fused && planes) {
+        if (planes && !geometry_done) {
             long pg = prof_begin(s, PROF_GEOMETRY);
-            HIPCHK(c, launch_extract_geometry(a, full, ms, s->vq, s->slot_layer_dev, s->nslots, zp, zn));
+            HIPCHK(c, launch_extract_geometry(a, full, ms, nullptr, nullptr, 0, zp, zn));
             prof_end(s, pg);
-            geometry_done = true;
-        } else if (fused) {
-            long pp = prof_begin(s, PROF_PHASE);
-            HIPCHK(c, phase(ms));
-            prof_end(s, pp);
         }
-        for (long step = 0; step < nsteps; step++) {
-            long pr = prof_begin(s, step_kind(fused, step, G));
-            HIPCHK(c, launch_step(s, fused, step, sa, ms));
-            prof_end(s, pr);
-        }
+        if (clear_sums && !planes) HIPCHK(c, launch_zero_f64(s->bsum, zn, ms));
     }
-    if (planes && !geometry_done) {
-        long pg = prof_begin(s, PROF_GEOMETRY);
-        HIPCHK(c, launch_extract_geometry(a, full, ms, nullptr, nullptr, 0, zp, zn));
-        prof_end(s, pg);
-    }
-    if (clear_sums && !planes) HIPCHK(c, launch_zero_f64(s->bsum, zn, ms));
+    if (!(part & FRAME_EXTRACT)) return ATSIM_OK;
     long pe = prof_begin(s, PROF_EXTRACT);
     HIPCHK(c, launch_extract(a, full, ms));
     prof_end(s, pe);
@@ -1470,10 +1588,69 @@ static int enqueue_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field
     return ATSIM_OK;
 }
 
+// Capture one part of a frame enqueued on st into an instantiated graph.
+static int capture_frame(atsim_sim* s, hipStream_t st, int part, float* d_pha, float* d_amp, void* d_field,
+                         hipGraph_t* graph, hipGraphExec_t* exec) {
+    atsim_ctx* c = s->ctx;
+    HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_frame(s, d_pha, d_amp, d_field, st, part);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (r) {
+        if (g) hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, ATSIM_EHIP, "frame graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        hipGraphDestroy(g);
+        return fail(c, ATSIM_EHIP, "frame graph instantiate: %s", hipGetErrorString(e));
+    }
+    *graph = g;
+    return ATSIM_OK;
+}
+
+// The captured graphs of a frame with these outputs on buffer set p (split:
+// a pipelined frame's two parts), captured now into the least recently used
+// slot when there are none.
+static int frame_graphs(atsim_sim* s, float* d_pha, float* d_amp, void* d_field, bool split,
+                        atsim_sim::GraphSlot** out) {
+    atsim_ctx* c = s->ctx;
+    atsim_sim::GraphSlot* gs = nullptr;
+    for (auto& g : s->gslot)
+        if (g.exec && g.pha == d_pha && g.amp == d_amp && g.field == d_field && g.parity == s->parity &&
+            g.split == split)
+            gs = &g;
+    if (!gs) {
+        gs = &s->gslot[0];
+        for (auto& g : s->gslot)
+            if (g.used < gs->used) gs = &g;
+        drop_graph_slot(*gs);
+        gs->used = -1;
+        if (split) {
+            if (int r = capture_frame(s, s->hs, FRAME_PASSES, d_pha, d_amp, d_field, &gs->graph, &gs->exec)) return r;
+            if (int r = capture_frame(s, c->stream, FRAME_EXTRACT, d_pha, d_amp, d_field, &gs->graph2, &gs->exec2)) {
+                drop_graph_slot(*gs);
+                return r;
+            }
+        } else if (int r = capture_frame(s, c->stream, FRAME_ALL, d_pha, d_amp, d_field, &gs->graph, &gs->exec)) {
+            return r;
+        }
+        gs->pha = d_pha;
+        gs->amp = d_amp;
+        gs->field = d_field;
+        gs->parity = s->parity;
+        gs->split = split;
+    }
+    gs->used = s->frames_done;
+    *out = gs;
+    return ATSIM_OK;
+}
+
 int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     if (!s || !d_pha || !d_amp) return ATSIM_EINVAL;
     atsim_ctx* c = s->ctx;
-    if (int r = advance_positions(s)) return r;
     // graph mode: from the second frame on (the first one sets the kernels'
     // LDS attributes), replay the captured kernel sequence; per-launch
     // profiling needs the eager path
@@ -1486,37 +1663,51 @@ int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
         s->graph_gen = c->tuning_gen;
         retuned = true;
     }
-    if (!c->graph || s->prof || s->frames_done == 0 || retuned) {
+    const bool eager = !c->graph || s->prof || s->frames_done == 0 || retuned;
+    const bool pipe = c->pipeline == 1 && s->propagate && !s->prof;
+    if (!pipe) {
+        // the whole frame on the context stream; a later pipelined frame's
+        // passes wait for it
+        s->head_sync = true;
+        if (int r = advance_positions(s, c->stream)) return r;
+        if (eager) {
+            s->frames_done++;
+            return enqueue_frame(s, d_pha, d_amp, d_field);
+        }
+        atsim_sim::GraphSlot* gs = nullptr;
+        if (int r = frame_graphs(s, d_pha, d_amp, d_field, false, &gs)) return r;
+        HIPCHK(c, hipGraphLaunch(gs->exec, c->stream));
         s->frames_done++;
-        return enqueue_frame(s, d_pha, d_amp, d_field);
+        return ATSIM_OK;
     }
+    // Pipelined: the passes on the simulation's stream hs into buffer set p,
+    // the extraction and fix-up on the context stream after them.  Set p was
+    // last read by the extraction of two frames back, so the passes wait for
+    // it; they wait for the whole context stream when something other than
+    // this simulation's frames wrote device memory there since its last frame.
+    if (int r = pipeline_init(s)) return r;
+    const int p = s->parity ^ 1;
+    use_buffers(s, p);
+    hipStream_t hs = s->hs, ts = c->stream;
+    if (s->tail_rec[p]) HIPCHK(c, hipStreamWaitEvent(hs, s->ev_tail[p], 0));
+    if (s->head_sync || s->seen_gen != c->work_gen) {
+        HIPCHK(c, hipEventRecord(s->ev_sync, ts));
+        HIPCHK(c, hipStreamWaitEvent(hs, s->ev_sync, 0));
+        s->head_sync = false;
+        s->seen_gen = c->work_gen;
+    }
+    if (int r = advance_positions(s, hs)) return r;
     atsim_sim::GraphSlot* gs = nullptr;
-    for (auto& g : s->gslot)
-        if (g.exec && g.pha == d_pha && g.amp == d_amp && g.field == d_field) gs = &g;
-    if (!gs) {
-        gs = s->gslot[0].used <= s->gslot[1].used ? &s->gslot[0] : &s->gslot[1];
-        drop_graph_slot(*gs);
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int r = enqueue_frame(s, d_pha, d_amp, d_field);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (r) {
-            if (g) hipGraphDestroy(g);
-            return r;
-        }
-        if (e != hipSuccess) return fail(c, ATSIM_EHIP, "frame graph capture: %s", hipGetErrorString(e));
-        gs->graph = g;
-        e = hipGraphInstantiate(&gs->exec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            drop_graph_slot(*gs);
-            return fail(c, ATSIM_EHIP, "frame graph instantiate: %s", hipGetErrorString(e));
-        }
-        gs->pha = d_pha;
-        gs->amp = d_amp;
-        gs->field = d_field;
-    }
-    gs->used = s->frames_done;
-    HIPCHK(c, hipGraphLaunch(gs->exec, c->stream));
+    if (!eager)
+        if (int r = frame_graphs(s, d_pha, d_amp, d_field, true, &gs)) return r;
+    if (gs) HIPCHK(c, hipGraphLaunch(gs->exec, hs));
+    else if (int r = enqueue_frame(s, d_pha, d_amp, d_field, hs, FRAME_PASSES)) return r;
+    HIPCHK(c, hipEventRecord(s->ev_head[p], hs));
+    HIPCHK(c, hipStreamWaitEvent(ts, s->ev_head[p], 0));
+    if (gs) HIPCHK(c, hipGraphLaunch(gs->exec2, ts));
+    else if (int r = enqueue_frame(s, d_pha, d_amp, d_field, ts, FRAME_EXTRACT)) return r;
+    HIPCHK(c, hipEventRecord(s->ev_tail[p], ts));
+    s->tail_rec[p] = true;
     s->frames_done++;
     return ATSIM_OK;
 }
@@ -1541,13 +1732,14 @@ int atsim_sims_frame(atsim_sim** sims, long n, float** d_pha, float** d_amp) {
     }
     if (!c->mg_fork) HIPCHK(c, hipEventCreateWithFlags(&c->mg_fork, hipEventDisableTiming));
     for (long k = 0; k < n; k++)
-        if (int r = advance_positions(sims[k])) return r;
+        if (int r = advance_positions(sims[k], c->stream)) return r;
     // every simulation's frame on its own stream, forked from and joined to
     // the context stream: frames of different grid sizes run side by side.
     // The launches are eager (this path captures no graph).
     HIPCHK(c, hipEventRecord(c->mg_fork, c->stream));
     for (long k = 0; k < n; k++) {
         atsim_sim* s = sims[k];
+        s->head_sync = true;  // a later pipelined frame's passes wait for this one
         HIPCHK(c, hipStreamWaitEvent(s->own, c->mg_fork, 0));
         if (int r = enqueue_frame(s, d_pha[k], d_amp[k], nullptr, s->own)) return r;
         HIPCHK(c, hipEventRecord(s->own_done, s->own));
@@ -1559,6 +1751,8 @@ int atsim_sims_frame(atsim_sim** sims, long n, float** d_pha, float** d_amp) {
 
 int atsim_sim_use_masters(atsim_sim* s, const atsim_sim* owner) {
     if (!s || !owner || s == owner) return ATSIM_EINVAL;
+    s->head_sync = true;
+    s->ctx->work_gen++;
     atsim_ctx* c = s->ctx;
     if (owner->ctx != c || owner->M != s->M || owner->L != s->L)
         return fail(c, ATSIM_EINVAL, "use_masters: the owner needs the same context, M and L");
@@ -1572,6 +1766,8 @@ int atsim_sim_use_masters(atsim_sim* s, const atsim_sim* owner) {
 
 int atsim_sim_synth_masters(atsim_sim* s, uint64_t seed, const double* outerscale_px, const double* innerscale_px) {
     if (!s || !outerscale_px || !innerscale_px) return ATSIM_EINVAL;
+    s->head_sync = true;
+    s->ctx->work_gen++;
     atsim_ctx* c = s->ctx;
     if (!supported_n(s->M)) return fail(c, ATSIM_EUNSUPPORTED, "synth_masters: M=%ld unsupported", s->M);
     float2* noise = nullptr;
@@ -1591,6 +1787,8 @@ int atsim_sim_synth_masters(atsim_sim* s, uint64_t seed, const double* outerscal
 int atsim_sim_normalize_masters(atsim_sim* s, double r0_goal_px, const double* cn2, double zangle,
                                 double* r0_measured) {
     if (!s || !cn2 || !(r0_goal_px > 0)) return ATSIM_EINVAL;
+    s->head_sync = true;
+    s->ctx->work_gen++;
     atsim_ctx* c = s->ctx;
     const long dpix = 50 < s->M / 2 ? 50 : s->M / 4;
     double r0tot = 0.0;
@@ -1649,6 +1847,7 @@ const char* tune_name(int k) {
 
 int atsim_sim_autotune(atsim_sim* s, float* d_pha, float* d_amp, int frames, char* report, long report_len) {
     if (!s || !d_pha || !d_amp || frames <= 0 || frames > 1000) return ATSIM_EINVAL;
+    s->head_sync = true;
     atsim_ctx* c = s->ctx;
     if (s->prof) return fail(c, ATSIM_EINVAL, "autotune: per-launch profiling is on");
     const long N = s->N, W = s->W;
@@ -1740,6 +1939,7 @@ int atsim_sim_autotune(atsim_sim* s, float* d_pha, float* d_amp, int frames, cha
 
 int atsim_sim_profile(atsim_sim* s, int enable) {
     if (!s) return ATSIM_EINVAL;
+    s->head_sync = true;
     HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
     s->prof = enable != 0;
     s->ev_used = 0;
@@ -1768,6 +1968,7 @@ int atsim_sim_profile_read(atsim_sim* s, int kind, double* ms, long* count) {
 // ------------------------------------------------------------------ layer screens
 int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
     if (!s || !out) return ATSIM_EINVAL;
+    s->head_sync = true;
     atsim_ctx* c = s->ctx;
     const long F = s->D * s->W, need = F * s->xs * s->xs * 2;
     if (n != need) return fail(c, ATSIM_EINVAL, "block sums: n = %ld, expected %ld", n, need);
@@ -1782,6 +1983,7 @@ int atsim_sim_block_sums(atsim_sim* s, double* out, long n) {
 
 int atsim_sim_stats_enable(atsim_sim* s, int enable) {
     if (!s) return ATSIM_EINVAL;
+    s->head_sync = true;
     atsim_ctx* c = s->ctx;
     const long F = s->D * s->W;
     if (enable && !s->st_out) {
@@ -1821,6 +2023,7 @@ int atsim_sim_stats(atsim_sim* s, double* out, long n) {
 
 int atsim_block_fixup(atsim_ctx* c, const double* bsum, long xs, double* peaks) {
     if (!c || !bsum || !peaks || xs <= 0 || xs > 4096) return ATSIM_EINVAL;
+    c->work_gen++;
     const size_t nb = (size_t)xs * xs;
     double *d_sum = nullptr, *d_bin = nullptr, *d_ch = nullptr;
     unsigned* d_max = nullptr;
@@ -1851,6 +2054,7 @@ int atsim_block_fixup(atsim_ctx* c, const double* bsum, long xs, double* peaks) 
 
 int atsim_block_counts(atsim_ctx* c, const double* peaks, long xs, float* counts) {
     if (!c || !peaks || !counts || xs <= 0 || xs > 4096) return ATSIM_EINVAL;
+    c->work_gen++;
     const size_t nb = (size_t)xs * xs;
     double* d_bin = nullptr;
     float* d_ch = nullptr;
@@ -1880,6 +2084,7 @@ int atsim_block_counts(atsim_ctx* c, const double* peaks, long xs, float* counts
 
 int atsim_sim_layer_screens(atsim_sim* s, float* d_out) {
     if (!s || !d_out) return ATSIM_EINVAL;
+    s->head_sync = true;
     atsim_ctx* c = s->ctx;
     HIPCHK(c, launch_layer_screens(s->masters, s->ls_dev, d_out, s->N, s->M, s->L, s->D, c->stream));
     return ATSIM_OK;
@@ -1897,6 +2102,7 @@ int atsim_gemm_bf16_ex(atsim_ctx* c, long M, long N, long K, long batch, float a
                        long sam, long sak, const float* B, long sbb, long sbk, long sbn, float beta, float* C,
                        long scb, long scm, long scn, const float* bias, float* pre, int gelu) {
     if (!c || !A || !B || !C || M < 0 || N < 0 || K < 0 || batch < 0) return ATSIM_EINVAL;
+    c->work_gen++;
     if (sab < 0 || sam < 0 || sak < 0 || sbb < 0 || sbk < 0 || sbn < 0 || scb < 0 || scm < 0 || scn < 0)
         return fail(c, ATSIM_EINVAL, "gemm: negative stride");
     GemmArgs g{M, N, K, batch, alpha, beta, A, sab, sam, sak, B, sbb, sbk, sbn, C, scb, scm, scn, bias, pre,
@@ -1910,18 +2116,21 @@ int atsim_gemm_bf16_ex(atsim_ctx* c, long M, long N, long K, long batch, float a
 int atsim_nn_add_bias_act(atsim_ctx* c, const float* a, const float* b, const float* bias, float* pre, float* y,
                           long C, long P, long n, int gelu) {
     if (!c || !a || !y || C <= 0 || P <= 0 || n < 0 || n % (C * P)) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_add_bias_act(a, b, bias, pre, y, C, P, n, gelu, c->stream));
     return ATSIM_OK;
 }
 
 int atsim_nn_gelu_bwd(atsim_ctx* c, const float* pre, const float* dy, float* dx, long n) {
     if (!c || !pre || !dy || !dx || n < 0) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_gelu_bwd(pre, dy, dx, n, c->stream));
     return ATSIM_OK;
 }
 
 int atsim_nn_bias_grad(atsim_ctx* c, const float* dy, float* db, long S, long C, long P, int accumulate) {
     if (!c || !dy || !db || S <= 0 || C <= 0 || P <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     const int nchunk = 64;
     const size_t need = sizeof(float) * nchunk * C;
     if (c->nn_bias_cap < need) {
@@ -1938,6 +2147,7 @@ int atsim_nn_bias_grad(atsim_ctx* c, const float* dy, float* db, long S, long C,
 
 int atsim_nn_mse(atsim_ctx* c, const float* pred, const float* target, float* grad, long n, double* loss) {
     if (!c || !pred || !target || n <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     const int nblk = 256;
     if (!c->nn_part) HIPCHK(c, hipMalloc(&c->nn_part, sizeof(double) * nblk));
     HIPCHK(c, launch_mse(pred, target, grad, n, c->nn_part, nblk, c->stream));
@@ -1952,6 +2162,7 @@ int atsim_nn_mse(atsim_ctx* c, const float* pred, const float* target, float* gr
 
 int atsim_nn_reduce_batches(atsim_ctx* c, const float* part, float* out, long nb, long n, int accumulate) {
     if (!c || !part || !out || nb <= 0 || n <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_reduce_batches(part, out, nb, n, accumulate, c->stream));
     return ATSIM_OK;
 }
@@ -1960,6 +2171,7 @@ int atsim_nn_make_pairs(atsim_ctx* c, const float* screens, const double* d_scoe
                         const float* amp, long D, long L, long N, float* x, float* t, long Stot, long s0, double xs,
                         double xl, double cs, double ps) {
     if (!c || !screens || !d_scoeff || !x || D <= 0 || L <= 0 || N <= 1 || s0 < 0 || s0 + D > Stot) return ATSIM_EINVAL;
+    c->work_gen++;
     if (!(xs > 0) || !(xl > 0) || !(cs > 0) || !(ps > 0)) return fail(c, ATSIM_EINVAL, "make_pairs: scales must be > 0");
     HIPCHK(c, launch_nn_pairs(screens, d_scoeff, pha, amp, D, L, N, Stot, s0, x, t, xs, xl, cs, ps, c->stream));
     return ATSIM_OK;
@@ -1968,6 +2180,7 @@ int atsim_nn_make_pairs(atsim_ctx* c, const float* screens, const double* d_scoe
 int atsim_nn_unpack(atsim_ctx* c, const float* y, const float* x, long L, long S, long P, double xs, double cs,
                     double ps, float* amp, float* pha) {
     if (!c || !y || !x || !amp || !pha || L <= 0 || S <= 0 || P <= 0) return ATSIM_EINVAL;
+    c->work_gen++;
     HIPCHK(c, launch_nn_unpack(y, x, L, S, P, xs, cs, ps, amp, pha, c->stream));
     return ATSIM_OK;
 }
@@ -1975,6 +2188,7 @@ int atsim_nn_unpack(atsim_ctx* c, const float* y, const float* x, long L, long S
 int atsim_nn_adam(atsim_ctx* c, float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2,
                   float eps, long step) {
     if (!c || !w || !g || !m || !v || n < 0 || step < 1) return ATSIM_EINVAL;
+    c->work_gen++;
     const float c1 = (float)(1.0 - pow((double)b1, (double)step)), c2 = (float)(1.0 - pow((double)b2, (double)step));
     HIPCHK(c, launch_adam(w, g, m, v, n, lr, b1, b2, eps, c1, c2, c->stream));
     return ATSIM_OK;

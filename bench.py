@@ -16,7 +16,9 @@ its launches inside the timed region; cpu_baseline = the CPU oracle on a
 bounded sample of the same workload (one wavelength per host thread), which
 also gives the FP64 error of every full-size field; pcie_inclusive = the rate
 with each frame handed to pinned host memory; per_wavelength_plain = the
-same frames from one simulation per wavelength.
+same frames from one simulation per wavelength; pipelined = both rates again
+with frame k + 1's passes beside frame k's extraction (tuning "pipeline"),
+timed last.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -452,6 +454,10 @@ def main():
                     "(atsim_sim_autotune) before the timed frames; 0: built-in defaults or the flags above")
     ap.add_argument("--graph", type=int, default=1, help="1: frames replay a captured HIP graph of their kernels "
                     "(0: eager launches)")
+    ap.add_argument("--pipeline", type=int, default=0, help="1: frame k + 1's Fresnel passes run beside frame k's "
+                    "extraction and block fix-up in the timed frames (two buffer sets; 0: frames one after the other)")
+    ap.add_argument("--pipelined", type=int, default=1, help="1 (with --pipeline 0): time the same frames and the "
+                    "per-wavelength ones pipelined at the end of the run ('pipelined' in the line)")
     ap.add_argument("--host-copy-steps", type=int, default=20,
                     help="extra frames each followed by a synchronous copy of OPD + amplitude to pinned host "
                          "memory (the PCIe-inclusive rate of a host hand-over; 0 = skip)")
@@ -513,6 +519,7 @@ def main():
     sim.synth_masters(wl["seed"], wl["L0"] / wl["dx"], wl["l0"] / wl["dx"])
     r0_meas = sim.normalize_masters(wl["r0"] / wl["dx"], wl["cn2"])
     t_setup = time.perf_counter() - t0
+    ctx.set_tuning("pipeline", args.pipeline)
     tune = None
     if args.autotune:
         t0 = time.perf_counter()
@@ -602,8 +609,6 @@ def main():
                 s1.frame(sync=False)
         ctx.sync()
         el = time.perf_counter() - t0
-        for s1 in sims1:
-            s1.close()
         v1 = args.plain_steps * wl["W"] / el
         per_lambda = {"value": round(v1, 2), "unit": "wavefront-frames/s", "steps": args.plain_steps,
                       "how": "one simulation per wavelength, one field per launch, same masters and kernels"}
@@ -612,6 +617,38 @@ def main():
     mixed = None
     if args.mixed_steps > 0:
         mixed = mixed_sizes(ctx, wl, sim, args.mixed_steps)
+    pipelined = None
+    if args.pipelined and not args.pipeline:
+        # the same frames with frame k + 1's passes beside frame k's
+        # extraction and fix-up (tuning "pipeline"), last, so that its extra
+        # stream cannot change how the measurements above share the GPU's
+        # hardware queues
+        ctx.set_tuning("pipeline", 1)
+
+        def rate(sims, steps):
+            for s_ in sims:
+                for _ in range(3):
+                    s_.frame(sync=False)
+            ctx.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                for s_ in sims:
+                    s_.frame(sync=False)
+            ctx.sync()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        el = rate([sim], args.steps)
+        pipelined = {"value": round(args.steps * F / el, 2), "unit": "wavefront-frames/s", "steps": args.steps,
+                     "ms_per_step": round(el / args.steps * 1e3, 4)}
+        if per_lambda is not None:
+            el = rate(sims1, args.plain_steps)
+            pipelined["per_wavelength_plain"] = round(args.plain_steps * wl["W"] / el, 2)
+        ctx.set_tuning("pipeline", 0)
+    if per_lambda is not None:
+        for s1 in sims1:
+            s1.close()
 
     N = wl["N"]
     kinds = ("row", "col", "row2", "col2", "phase")
@@ -740,12 +777,13 @@ def main():
         "per_wavelength_plain": per_lambda,
         "batched_vs_per_wavelength": round(value / per_lambda["value"], 2) if per_lambda else None,
         "mixed_sizes": mixed,
+        "pipelined": pipelined,
         "fft_variants": variants,
         "setup_s": round(t_setup, 3),
         "tuning": {"row_mw": ctx.get_tuning("row_mw"), "col_c": ctx.get_tuning("col_c"),
                    "col_e": ctx.get_tuning("col_e"), "streams": ctx.get_tuning("streams"),
                    "gap_passes": ctx.get_tuning("gap_passes"), "col2_e": ctx.get_tuning("col2_e"),
-                   "graph": ctx.get_tuning("graph")},
+                   "graph": ctx.get_tuning("graph"), "pipeline": ctx.get_tuning("pipeline")},
         "autotune": tune,
         "r0_masters_px_before_norm": [round(float(v), 3) for v in r0_meas],
     }

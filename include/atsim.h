@@ -102,6 +102,23 @@ int atsim_stream_wait_copy(atsim_ctx *ctx, int slot);
  *             visits only the blocks where a count can be non-zero (from
  *             each block's largest |r|, kept by the extraction); 2: every
  *             pixel.  Both give the same bits.
+ *   "pipeline" 1: frames of a simulation overlap.  Frame k + 1's Fresnel
+ *             passes run on a stream of the simulation's own beside frame
+ *             k's extraction and block fix-up, which stay on the context
+ *             stream after the passes of their frame; two sets of fields,
+ *             planes and block sums alternate (allocated at the first such
+ *             frame).  A frame's outputs are complete when the context
+ *             stream reaches them, and are the same bits as with 0 (the
+ *             default: every frame on the context stream).  The passes wait
+ *             for the context stream when an atsim call that writes device
+ *             memory came since the simulation's previous frame; device
+ *             writes of the caller's own to the masters (through
+ *             atsim_sim_master's pointer) must be complete before the next
+ *             atsim_sim_frame call.  Profiled frames run without it.  Leave
+ *             it at 0 in a loop that hands every frame to the host through
+ *             atsim_copy_async: the copy stream can share a hardware queue
+ *             with the passes' stream, whose next frame then waits for the
+ *             copy.
  * Returns 0 / < 0; atsim_get_tuning returns the value or < 0. */
 int atsim_set_tuning(atsim_ctx *ctx, const char *key, int value);
 int atsim_get_tuning(atsim_ctx *ctx, const char *key);

@@ -178,6 +178,64 @@ def test_extraction_from_geometric_phase_planes(oracle, streams):
     cx.close()
 
 
+@pytest.mark.parametrize("graph,streams", [(0, 2), (1, 1), (1, 2)])
+def test_pipelined_frames(oracle, graph, streams):
+    # tuning "pipeline" 1: frame k + 1's passes run on the simulation's own
+    # stream beside frame k's extraction and fix-up (two buffer sets
+    # alternate).  Every frame's outputs are the same bits as without it,
+    # through alternating output buffers, a field output, a master replaced
+    # between frames, a multi-simulation frame in the sequence and per-launch
+    # profiling; both match the oracle.
+    from atmosphericturbulencesimul_amd.api import Context, sims_frame
+    lams = [0.5e-6, 0.8e-6, 1.25e-6, 2.2e-6]
+    c = _cfg1_like(oracle, N=128, Nout=100, M=256, lambdas=lams, unwrap_mode=[0, 1, 1, 1])
+    print(f"case: 128^2 -> 100^2, 3 layers, 4 wavelengths, graph {graph}, streams {streams}, 9 frames", flush=True)
+    runs = []
+    for pipe in (0, 1):
+        cx = Context(0)
+        cx.set_tuning("graph", graph)
+        cx.set_tuning("streams", streams)
+        cx.set_tuning("pipeline", pipe)
+        assert cx.get_tuning("pipeline") == pipe
+        gs = gpu_sim(cx, c)
+        outs = [(torch.empty_like(gs.pha), torch.empty_like(gs.amp)) for _ in range(2)]
+        field = torch.empty((c.D, c.W, c.N, c.N), dtype=torch.complex64, device="cuda")
+        m1 = gs.master(1) * 0.5
+        got = []
+        for k in range(9):
+            f = None
+            if k == 3:
+                gs.set_master(1, m1)
+            if k == 5:
+                sims_frame([gs], sync=False)
+                pha, amp = gs.pha, gs.amp
+            elif k == 6:
+                gs.profile(True)
+                pha, amp = gs.frame(sync=False, out=outs[k % 2])
+                gs.profile(False)
+            else:
+                f = field if k == 4 else None
+                pha, amp = gs.frame(f, sync=False, out=outs[k % 2])
+            # copies ordered after the frame on the library's stream
+            with torch.cuda.stream(cx.stream):
+                got.append((pha.clone(), amp.clone(), None if f is None else f.clone()))
+        cx.sync()
+        torch.cuda.synchronize()
+        runs.append([tuple(None if t is None else t.cpu().numpy() for t in g) for g in got])
+        gs.close()
+        cx.close()
+    for k, (a, b) in enumerate(zip(*runs)):
+        for x, y in zip(a, b):
+            if x is not None:
+                assert np.array_equal(x, y), f"frame {k}"
+    cx = Context(0)
+    cx.set_tuning("graph", graph)
+    cx.set_tuning("streams", streams)
+    cx.set_tuning("pipeline", 1)
+    _compare(c, oracle_sims(oracle, c), gpu_sim(cx, c), nframes=3)
+    cx.close()
+
+
 def _block_sums_np(E, PHI, Nout):
     """sum cos / sin of PHI - arg E per (jj2, ii2) block over the centred
     Nout x Nout crop, blocks (long)(1.0 * i / Nout * xs) with xs = Nout / 20
