@@ -989,7 +989,13 @@ static void prof_end(atsim_sim* s, long i0) {
     (void)hipEventRecord(s->ev_pool[(size_t)i0 + 1], s->ctx->stream);
 }
 
-static void drop_graph_slot(atsim_sim::GraphSlot& g) {
+// Destroy a slot's graphs once no launch of them can still be running (the
+// frame streams are drained first; a slot is dropped only on a tuning or
+// output change, not per frame).
+static void drop_graph_slot(atsim_sim* s, atsim_sim::GraphSlot& g) {
+    if (!g.exec && !g.exec2) return;
+    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    if (s->hs) hipStreamSynchronize(s->hs);
     if (g.exec) hipGraphExecDestroy(g.exec);
     if (g.graph) hipGraphDestroy(g.graph);
     if (g.exec2) hipGraphExecDestroy(g.exec2);
@@ -998,7 +1004,7 @@ static void drop_graph_slot(atsim_sim::GraphSlot& g) {
 }
 
 static void sim_drop_graph(atsim_sim* s) {
-    for (auto& g : s->gslot) drop_graph_slot(g);
+    for (auto& g : s->gslot) drop_graph_slot(s, g);
 }
 
 static void sim_free(atsim_sim* s) {
@@ -1626,12 +1632,12 @@ static int frame_graphs(atsim_sim* s, float* d_pha, float* d_amp, void* d_field,
         gs = &s->gslot[0];
         for (auto& g : s->gslot)
             if (g.used < gs->used) gs = &g;
-        drop_graph_slot(*gs);
+        drop_graph_slot(s, *gs);
         gs->used = -1;
         if (split) {
             if (int r = capture_frame(s, s->hs, FRAME_PASSES, d_pha, d_amp, d_field, &gs->graph, &gs->exec)) return r;
             if (int r = capture_frame(s, c->stream, FRAME_EXTRACT, d_pha, d_amp, d_field, &gs->graph2, &gs->exec2)) {
-                drop_graph_slot(*gs);
+                drop_graph_slot(s, *gs);
                 return r;
             }
         } else if (int r = capture_frame(s, c->stream, FRAME_ALL, d_pha, d_amp, d_field, &gs->graph, &gs->exec)) {
