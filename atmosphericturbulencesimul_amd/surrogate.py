@@ -312,7 +312,9 @@ def main(argv=None):
     ap.add_argument("--judge-every", type=int, default=50)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--epochs-per-chunk", type=int, default=1)
+    # passes over each chunk of freshly generated pairs (scintillation-index
+    # error 23 % after one pass, 15 % after 4, 13 % after 8)
+    ap.add_argument("--epochs-per-chunk", type=int, default=8)
     ap.add_argument("--width", type=int, default=32)
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--dx", type=float, default=0.005)
