@@ -1617,9 +1617,9 @@ static int capture_frame(atsim_sim* s, hipStream_t st, int part, float* d_pha, f
     return ATSIM_OK;
 }
 
-// The captured graphs of a frame with these outputs on buffer set p (split:
-// a pipelined frame's two parts), captured now into the least recently used
-// slot when there are none.
+// The captured graphs of a frame with these outputs on the current buffer set
+// (split: a pipelined frame's two parts), captured now into the least
+// recently used slot when there are none.
 static int frame_graphs(atsim_sim* s, float* d_pha, float* d_amp, void* d_field, bool split,
                         atsim_sim::GraphSlot** out) {
     atsim_ctx* c = s->ctx;
@@ -1657,12 +1657,12 @@ static int frame_graphs(atsim_sim* s, float* d_pha, float* d_amp, void* d_field,
 int atsim_sim_frame(atsim_sim* s, float* d_pha, float* d_amp, void* d_field) {
     if (!s || !d_pha || !d_amp) return ATSIM_EINVAL;
     atsim_ctx* c = s->ctx;
-    // graph mode: from the second frame on (the first one sets the kernels'
-    // LDS attributes), replay the captured kernel sequence; per-launch
-    // profiling needs the eager path
-    // a tuning change since the graphs were captured: drop them, run this
-    // frame eagerly (first launches of other kernel variants set their LDS
-    // attributes outside a capture) and capture again at the next one
+    // Graph mode replays captured kernel sequences from the second frame on
+    // (the first frame sets the kernels' LDS attributes); per-launch
+    // profiling needs the eager path.  After a tuning change the graphs are
+    // dropped, this frame runs eagerly (first launches of other kernel
+    // variants set their LDS attributes outside a capture) and the next one
+    // captures again.
     bool retuned = false;
     if (s->graph_gen != c->tuning_gen) {
         sim_drop_graph(s);
